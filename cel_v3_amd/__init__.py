@@ -1,0 +1,387 @@
+"""cel_v3_amd -- MI355X (gfx950) Jacobi-PCG hot path of the CEL solver.
+
+The product is the C-ABI shared library cel_v3_amd/libcel_gpu.so
+(include/cel_gpu.h).  This module is a thin ctypes front end over it, used by
+the tests and bench.py; it has no compute path of its own and raises if the
+library is missing or a call fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcel_gpu.so")
+
+OK, NOT_CONVERGED = 0, 1
+ERR = {-1: "ARG", -2: "HIP", -3: "NO_DEVICE", -4: "BAD_MATRIX", -5: "BAD_DIAG", -6: "BREAKDOWN",
+       -7: "NONFINITE", -8: "OOM", -9: "REF_UNDEFINED", -10: "STATE"}
+FMT_CSR, FMT_JAD, FMT_COO = 1, 2, 3
+MEM_HOST, MEM_DEVICE = 0, 1
+X0_ONE, X0_ZERO, X0_USER = 0, 1, 2
+TOL_ABS, TOL_REL_B, TOL_REL_R0 = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+JAD_SPARSKIT, JAD_CEL_COMPAT = 0, 1
+
+# every entry point declared in include/cel_gpu.h
+EXPORTS = [
+    "cel_gpu_init", "cel_gpu_finalize", "cel_gpu_last_error", "cel_gpu_device_info",
+    "cel_gpu_malloc", "cel_gpu_free", "cel_gpu_memcpy_h2d", "cel_gpu_memcpy_d2h",
+    "cel_gpu_memcpy_d2d", "cel_gpu_synchronize", "cel_gpu_mat_create_csr",
+    "cel_gpu_mat_create_coo", "cel_gpu_mat_gen_poisson7", "cel_gpu_mat_gen_diffusion7",
+    "cel_gpu_mat_destroy", "cel_gpu_mat_info", "cel_gpu_mat_get_csr", "cel_gpu_mat_to_jad",
+    "cel_gpu_mat_get_jad", "cel_gpu_mat_set_jacobi", "cel_gpu_mat_scale_diagonal",
+    "cel_gpu_vec_scale_diagonal", "cel_gpu_mat_get_jacobi", "cel_gpu_mat_get_diagonal",
+    "cel_gpu_spmv", "cel_gpu_spmv_f32", "cel_gpu_spmv_time", "cel_gpu_mat_distr_split",
+    "cel_gpu_mat_get_block", "cel_gpu_mat_set_send_list", "cel_gpu_face_pack",
+    "cel_gpu_spmv_distr", "cel_gpu_mat_ghost_len", "cel_gpu_pcg_params_default",
+    "cel_gpu_pcg_solve", "cel_gpu_cg_adjoint", "cel_gpu_diffusion7_kappa_grad",
+]
+
+
+class CelError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERR.get(code, code)}: {msg}")
+        self.code = code
+
+
+class PcgParams(C.Structure):
+    _fields_ = [("tol", C.c_double), ("tol_mode", C.c_int), ("iter_max", C.c_int),
+                ("x0_mode", C.c_int), ("precision", C.c_int), ("format", C.c_int),
+                ("memspace", C.c_int), ("use_halo", C.c_int), ("ru_delta", C.c_double),
+                ("ru_restart", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int),
+                ("profile", C.c_int)]
+
+
+class PcgStats(C.Structure):
+    _fields_ = [("status", C.c_int), ("iters", C.c_int), ("outer_iters", C.c_int),
+                ("norm0", C.c_double), ("final_norm", C.c_double), ("eps_abs", C.c_double),
+                ("alpha", C.c_double), ("solve_ms", C.c_double), ("setup_ms", C.c_double),
+                ("spmv_ms", C.c_double), ("spmv_launches", C.c_int), ("spmv_bytes", C.c_double),
+                ("iter_bytes", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+_vp = C.c_void_p
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+
+
+def lib():
+    """Loads libcel_gpu.so.  Raises if it has not been built: there is no
+    other implementation to fall back to."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing; run `python -m cel_v3_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.cel_gpu_last_error.restype = C.c_char_p
+        L.cel_gpu_pcg_params_default.restype = None
+        L.cel_gpu_pcg_params_default.argtypes = [C.POINTER(PcgParams)]
+        for name in EXPORTS:
+            getattr(L, name)  # every declared symbol must be exported
+        _lib = L
+    return _lib
+
+
+def _check(rc, allow=(OK,)):
+    if rc in allow:
+        return rc
+    msg = lib().cel_gpu_last_error().decode(errors="replace")
+    raise CelError(rc, msg)
+
+
+def init(device=0):
+    _check(lib().cel_gpu_init(device))
+
+
+def device_info():
+    buf = C.create_string_buffer(256)
+    cus = C.c_int()
+    _check(lib().cel_gpu_device_info(buf, 256, C.byref(cus)))
+    return buf.value.decode(), cus.value
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---------------------------------------------------------------- device arrays
+class DeviceArray:
+    """A device allocation owned by the library's allocator."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.ptr = _vp()
+        _check(lib().cel_gpu_malloc(C.byref(self.ptr), C.c_size_t(max(1, self.n * self.dtype.itemsize))))
+
+    @classmethod
+    def from_host(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype or a.dtype)
+        d = cls(a.size, a.dtype)
+        _check(lib().cel_gpu_memcpy_h2d(d.ptr, _ptr(a), C.c_size_t(a.nbytes)))
+        return d
+
+    def to_host(self):
+        out = np.empty(self.n, self.dtype)
+        _check(lib().cel_gpu_memcpy_d2h(_ptr(out), self.ptr, C.c_size_t(out.nbytes)))
+        return out
+
+    def copy_from(self, other: "DeviceArray"):
+        _check(lib().cel_gpu_memcpy_d2d(self.ptr, other.ptr, C.c_size_t(self.n * self.dtype.itemsize)))
+
+    def free(self):
+        if self.ptr:
+            lib().cel_gpu_free(self.ptr)
+            self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- matrices
+class Matrix:
+    """Device-resident local block (+ halo blocks after a distributed split)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        n, nc, nnz, mm, nb = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(lib().cel_gpu_mat_info(self.h, C.byref(n), C.byref(nc), C.byref(nnz), C.byref(mm),
+                                      C.byref(nb)))
+        self.n, self.ncols, self.nnz, self.nblocks = n.value, nc.value, nnz.value, nb.value
+
+    # constructors -------------------------------------------------------
+    @classmethod
+    def from_csr(cls, xadj, cols, vals, ncols=None, index_base=1, row_offset=0, col_offset=0):
+        xadj, cols, vals = _i32(xadj), _i32(cols), _f64(vals)
+        n = xadj.size - 1
+        h = _vp()
+        _check(lib().cel_gpu_mat_create_csr(C.byref(h), n, n if ncols is None else ncols, cols.size,
+                                            _ptr(xadj), _ptr(cols), _ptr(vals), index_base,
+                                            row_offset, col_offset, MEM_HOST))
+        return cls(h)
+
+    @classmethod
+    def from_coo(cls, nrows, rows, cols, vals, ncols=None, index_base=1, row_offset=0, col_offset=0):
+        rows, cols, vals = _i32(rows), _i32(cols), _f64(vals)
+        h = _vp()
+        _check(lib().cel_gpu_mat_create_coo(C.byref(h), nrows, nrows if ncols is None else ncols,
+                                            rows.size, _ptr(rows), _ptr(cols), _ptr(vals),
+                                            index_base, row_offset, col_offset))
+        return cls(h)
+
+    @classmethod
+    def poisson7(cls, nx, ny, nz):
+        h = _vp()
+        _check(lib().cel_gpu_mat_gen_poisson7(C.byref(h), nx, ny, nz))
+        return cls(h)
+
+    @classmethod
+    def diffusion7(cls, nx, ny, nz, kappa):
+        h = _vp()
+        if isinstance(kappa, DeviceArray):
+            _check(lib().cel_gpu_mat_gen_diffusion7(C.byref(h), nx, ny, nz, kappa.ptr, MEM_DEVICE))
+        else:
+            k = _f64(kappa)
+            _check(lib().cel_gpu_mat_gen_diffusion7(C.byref(h), nx, ny, nz, _ptr(k), MEM_HOST))
+        return cls(h)
+
+    @classmethod
+    def distr_split(cls, vtxdist, my_rank, rows, cols, vals, index_base=1):
+        vtxdist, rows, cols, vals = _i32(vtxdist), _i32(rows), _i32(cols), _f64(vals)
+        h = _vp()
+        _check(lib().cel_gpu_mat_distr_split(C.byref(h), vtxdist.size - 1, _ptr(vtxdist), my_rank,
+                                             rows.size, _ptr(rows), _ptr(cols), _ptr(vals),
+                                             index_base))
+        return cls(h)
+
+    def destroy(self):
+        if self.h:
+            lib().cel_gpu_mat_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    # getters ------------------------------------------------------------
+    def get_csr(self):
+        xadj = np.empty(self.n + 1, np.int32)
+        cols = np.empty(max(self.nnz, 1), np.int32)
+        vals = np.empty(max(self.nnz, 1), np.float64)
+        _check(lib().cel_gpu_mat_get_csr(self.h, _ptr(xadj), _ptr(cols), _ptr(vals)))
+        return xadj, cols[: self.nnz], vals[: self.nnz]
+
+    def to_jad(self, mode=JAD_SPARSKIT, allow_undefined=False):
+        rc = lib().cel_gpu_mat_to_jad(self.h, mode)
+        _check(rc, allow=(OK, -9) if allow_undefined else (OK,))
+        return rc
+
+    def get_jad(self):
+        mm, nd, nw = C.c_int(), C.c_int(), C.c_longlong()
+        _check(lib().cel_gpu_mat_get_jad(self.h, None, None, None, None, C.byref(mm), C.byref(nd),
+                                         C.byref(nw)))
+        perm = np.empty(max(self.n, 1), np.int32)
+        begin = np.empty(mm.value + 1, np.int32)
+        cols = np.empty(max(nw.value, 1), np.int32)
+        vals = np.empty(max(nw.value, 1), np.float64)
+        _check(lib().cel_gpu_mat_get_jad(self.h, _ptr(perm), _ptr(begin), _ptr(cols), _ptr(vals),
+                                         C.byref(mm), C.byref(nd), C.byref(nw)))
+        return dict(perm=perm[: self.n], begin=begin, cols=cols[: nw.value], vals=vals[: nw.value],
+                    mmax=mm.value, ndefined=nd.value)
+
+    def set_jacobi(self, row_scaled=False):
+        _check(lib().cel_gpu_mat_set_jacobi(self.h, int(bool(row_scaled))))
+
+    def scale_diagonal(self):
+        _check(lib().cel_gpu_mat_scale_diagonal(self.h))
+
+    def scale_vec(self, b):
+        b = _f64(b).copy()
+        _check(lib().cel_gpu_vec_scale_diagonal(self.h, _ptr(b), MEM_HOST))
+        return b
+
+    def jacobi(self):
+        out = np.empty(self.n)
+        _check(lib().cel_gpu_mat_get_jacobi(self.h, _ptr(out)))
+        return out
+
+    def diagonal(self):
+        out = np.empty(self.n)
+        _check(lib().cel_gpu_mat_get_diagonal(self.h, _ptr(out)))
+        return out
+
+    # operators ----------------------------------------------------------
+    def spmv(self, x, y=None, fmt=FMT_CSR, accumulate=False):
+        if isinstance(x, DeviceArray):
+            _check(lib().cel_gpu_spmv(self.h, fmt, x.ptr, y.ptr, int(accumulate), MEM_DEVICE))
+            return y
+        x = _f64(x)
+        y = np.zeros(self.n) if y is None else _f64(y).copy()
+        _check(lib().cel_gpu_spmv(self.h, fmt, _ptr(x), _ptr(y), int(accumulate), MEM_HOST))
+        return y
+
+    def spmv_f32(self, x, y=None, fmt=FMT_CSR, accumulate=False):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros(self.n, np.float32) if y is None else np.ascontiguousarray(y, np.float32).copy()
+        _check(lib().cel_gpu_spmv_f32(self.h, fmt, _ptr(x), _ptr(y), int(accumulate), MEM_HOST))
+        return y
+
+    def spmv_time(self, x: DeviceArray, y: DeviceArray, fmt=FMT_CSR, precision=PREC_FP64, reps=20):
+        ms, nb = C.c_double(), C.c_double()
+        _check(lib().cel_gpu_spmv_time(self.h, fmt, precision, x.ptr, y.ptr, reps, C.byref(ms),
+                                       C.byref(nb)))
+        return ms.value, nb.value
+
+    # distributed --------------------------------------------------------
+    def get_block(self, b):
+        proc, nnz, rlen = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().cel_gpu_mat_get_block(self.h, b, C.byref(proc), C.byref(nnz), None, None, None,
+                                           None, C.byref(rlen), None))
+        k = nnz.value
+        rows = np.empty(max(k, 1), np.int32)
+        cols = np.empty(max(k, 1), np.int32)
+        vals = np.empty(max(k, 1), np.float64)
+        xadj = np.empty(self.n + 1, np.int32)
+        perm = np.empty(max(rlen.value, 1), np.int32)
+        _check(lib().cel_gpu_mat_get_block(self.h, b, C.byref(proc), C.byref(nnz), _ptr(rows),
+                                           _ptr(cols), _ptr(vals), _ptr(xadj), C.byref(rlen),
+                                           _ptr(perm)))
+        return dict(proc=proc.value, rows=rows[:k], cols=cols[:k], vals=vals[:k], xadj=xadj,
+                    recv_perm=perm[: rlen.value])
+
+    def set_send_list(self, peer, perm):
+        perm = _i32(perm)
+        _check(lib().cel_gpu_mat_set_send_list(self.h, peer, perm.size, _ptr(perm)))
+
+    def ghost_len(self):
+        g, s = C.c_int(), C.c_int()
+        _check(lib().cel_gpu_mat_ghost_len(self.h, C.byref(g), C.byref(s)))
+        return g.value, s.value
+
+    def face_pack(self, x):
+        _, st = self.ghost_len()
+        buf = np.empty(max(st, 1))
+        _check(lib().cel_gpu_face_pack(self.h, _ptr(_f64(x)), _ptr(buf), MEM_HOST))
+        return buf[:st]
+
+    def spmv_distr(self, x, ghost, fmt=FMT_CSR):
+        y = np.zeros(self.n)
+        g = _f64(ghost) if len(ghost) else np.zeros(1)
+        _check(lib().cel_gpu_spmv_distr(self.h, fmt, _ptr(_f64(x)), _ptr(g), _ptr(y), MEM_HOST))
+        return y
+
+    # solvers ------------------------------------------------------------
+    def pcg(self, b, x0=None, tol=1e-8, tol_mode=TOL_ABS, iter_max=1000, x0_mode=X0_ONE,
+            precision=PREC_FP64, fmt=FMT_CSR, history=False, allow_not_converged=True, **kw):
+        """Jacobi-PCG (cel_cgalg_jacobi).  b/x0 host arrays or DeviceArrays."""
+        p = params(tol=tol, tol_mode=tol_mode, iter_max=iter_max, x0_mode=x0_mode,
+                   precision=precision, format=fmt, **kw)
+        st = PcgStats()
+        hist = np.zeros(iter_max + 1) if history else None
+        hp = _ptr(hist) if history else None
+        if isinstance(b, DeviceArray):
+            p.memspace = MEM_DEVICE
+            x = x0 if x0 is not None else DeviceArray(self.n)
+            if x0 is not None:
+                p.x0_mode = X0_USER
+            rc = lib().cel_gpu_pcg_solve(self.h, b.ptr, x.ptr, C.byref(p), C.byref(st), hp)
+        else:
+            p.memspace = MEM_HOST
+            x = np.zeros(self.n) if x0 is None else _f64(x0).copy()
+            if x0 is not None:
+                p.x0_mode = X0_USER
+            rc = lib().cel_gpu_pcg_solve(self.h, _ptr(_f64(b)), _ptr(x), C.byref(p), C.byref(st), hp)
+        _check(rc, allow=(OK, NOT_CONVERGED) if allow_not_converged else (OK,))
+        out = st.as_dict()
+        out["x"] = x
+        if history:
+            out["hist"] = hist[: st.iters + 1]
+        return out
+
+    def adjoint(self, x, dLdx, want_dLda=True, tol=1e-12, tol_mode=TOL_REL_B, iter_max=10000, **kw):
+        p = params(tol=tol, tol_mode=tol_mode, iter_max=iter_max, x0_mode=X0_ZERO, **kw)
+        p.memspace = MEM_HOST
+        st = PcgStats()
+        lam = np.zeros(self.n)
+        g = np.zeros(max(self.nnz, 1)) if want_dLda else None
+        rc = lib().cel_gpu_cg_adjoint(self.h, _ptr(_f64(x)), _ptr(_f64(dLdx)), _ptr(lam),
+                                      _ptr(g) if want_dLda else None, C.byref(p), C.byref(st))
+        _check(rc, allow=(OK, NOT_CONVERGED))
+        return lam, (g[: self.nnz] if want_dLda else None), st.as_dict()
+
+
+def params(**kw) -> PcgParams:
+    p = PcgParams()
+    lib().cel_gpu_pcg_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise TypeError(f"unknown PCG parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+def kappa_grad(nx, ny, nz, kappa, x, lam):
+    g = np.empty(nx * ny * nz)
+    _check(lib().cel_gpu_diffusion7_kappa_grad(nx, ny, nz, _ptr(_f64(kappa)), _ptr(_f64(x)),
+                                               _ptr(_f64(lam)), _ptr(g), MEM_HOST))
+    return g

@@ -1,0 +1,914 @@
+// capi.hip -- host side of the C ABI declared in include/cel_gpu.h.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "cel_internal.hpp"
+
+struct cel_gpu_mat {
+    cel::Mat m;
+};
+
+namespace cel {
+
+int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel_pcg_stats& stats,
+              double* hist);
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(int code, const std::string& msg)
+{
+    set_error(msg);
+    return code;
+}
+
+Runtime& rt()
+{
+    static Runtime r;
+    return r;
+}
+
+int resident_grid(const void* kernel, long long work_items)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    long long g = (long long)rt().num_cus * per_cu;
+    if (g > work_items) g = work_items;
+    if (g < 1) g = 1;
+    if (g >= 8) g = g / 8 * 8;  // whole rounds over the 8 XCDs
+    return (int)g;
+}
+
+int ensure_init()
+{
+    if (rt().ready) return CEL_OK;
+    return cel_gpu_init(-1);
+}
+
+}  // namespace cel
+
+using namespace cel;
+
+// ------------------------------------------------------------------ runtime
+extern "C" int cel_gpu_init(int device)
+{
+    Runtime& R = rt();
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return fail(CEL_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0) device = R.ready ? R.device : 0;
+    if (device >= count) return fail(CEL_ERR_ARG, "device index out of range");
+    if (R.ready && R.device == device) return CEL_OK;
+    CEL_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    CEL_HIP(hipGetDeviceProperties(&prop, device));
+    if (R.stream) (void)hipStreamDestroy(R.stream);
+    // default priority stream, non-blocking w.r.t. the null stream
+    CEL_HIP(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking));
+    R.device = device;
+    R.num_cus = prop.multiProcessorCount;
+    R.ready = true;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_finalize(void)
+{
+    Runtime& R = rt();
+    if (!R.ready) return CEL_OK;
+    (void)hipStreamSynchronize(R.stream);
+    (void)hipStreamDestroy(R.stream);
+    R.stream = nullptr;
+    R.ready = false;
+    return CEL_OK;
+}
+
+extern "C" const char* cel_gpu_last_error(void) { return g_err.c_str(); }
+
+extern "C" int cel_gpu_device_info(char* name, int name_len, int* num_cus)
+{
+    CEL_TRY(ensure_init());
+    hipDeviceProp_t prop;
+    CEL_HIP(hipGetDeviceProperties(&prop, rt().device));
+    if (name && name_len > 0) {
+        std::string s = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+        std::strncpy(name, s.c_str(), (size_t)name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (num_cus) *num_cus = prop.multiProcessorCount;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_malloc(void** ptr, size_t bytes)
+{
+    CEL_TRY(ensure_init());
+    if (!ptr) return fail(CEL_ERR_ARG, "null pointer");
+    if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+        *ptr = nullptr;
+        return fail(CEL_ERR_OOM, "hipMalloc failed");
+    }
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_free(void* ptr)
+{
+    if (ptr) CEL_HIP(hipFree(ptr));
+    return CEL_OK;
+}
+
+static int copy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
+{
+    CEL_TRY(ensure_init());
+    if (bytes == 0) return CEL_OK;
+    CEL_HIP(hipMemcpyAsync(dst, src, bytes, kind, rt().stream));
+    CEL_HIP(hipStreamSynchronize(rt().stream));
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_memcpy_h2d(void* d, const void* s, size_t b) { return copy(d, s, b, hipMemcpyHostToDevice); }
+extern "C" int cel_gpu_memcpy_d2h(void* d, const void* s, size_t b) { return copy(d, s, b, hipMemcpyDeviceToHost); }
+extern "C" int cel_gpu_memcpy_d2d(void* d, const void* s, size_t b) { return copy(d, s, b, hipMemcpyDeviceToDevice); }
+
+extern "C" int cel_gpu_synchronize(void)
+{
+    CEL_TRY(ensure_init());
+    CEL_HIP(hipStreamSynchronize(rt().stream));
+    return CEL_OK;
+}
+
+// ------------------------------------------------------------------ matrices
+template <typename T>
+static int upload(DevBuf<T>& buf, const T* src, size_t n)
+{
+    CEL_TRY(buf.alloc(n > 0 ? n : 1));
+    if (n) CEL_HIP(hipMemcpyAsync(buf.p, src, n * sizeof(T), hipMemcpyHostToDevice, rt().stream));
+    return CEL_OK;
+}
+
+// Uploads a host CSR given as 0-based local xadj/cols.
+static int mat_from_host(Mat& M, int n, int ncols, const std::vector<int>& xadj0,
+                         const std::vector<int>& cols0, const double* vals, int nnz)
+{
+    M.n = n;
+    M.ncols = ncols;
+    M.nnz = nnz;
+    CEL_TRY(upload(M.xadj, xadj0.data(), (size_t)n + 1));
+    CEL_TRY(upload(M.cols, cols0.data(), (size_t)nnz));
+    CEL_TRY(upload(M.vals, vals, (size_t)nnz));
+    CEL_HIP(hipStreamSynchronize(rt().stream));
+    CEL_TRY(launch_extract_diag(M));
+    M.has_diag = true;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_mat_create_csr(cel_gpu_mat** A, int nrows, int ncols, int nnz,
+                                      const int* xadj, const int* cols, const double* vals,
+                                      int index_base, int row_offset, int col_offset, int memspace)
+{
+    CEL_TRY(ensure_init());
+    if (!A || nrows < 0 || ncols < 0 || nnz < 0 || (index_base != 0 && index_base != 1))
+        return fail(CEL_ERR_ARG, "bad arguments to cel_gpu_mat_create_csr");
+    if (nnz > 0 && (!cols || !vals)) return fail(CEL_ERR_ARG, "null arrays");
+    if (!xadj) return fail(CEL_ERR_ARG, "null xadj");
+    auto* h = new cel_gpu_mat();
+    Mat& M = h->m;
+    M.row_offset = row_offset;
+    M.col_offset = col_offset;
+    if (memspace == CEL_MEM_DEVICE) {
+        // device arrays: copied as given (0-based local columns expected when
+        // index_base == 0 and col_offset == 0); no validation pass
+        if (index_base != 0 || col_offset != 0) {
+            delete h;
+            return fail(CEL_ERR_ARG, "device CSR input must be 0-based with col_offset 0");
+        }
+        M.n = nrows;
+        M.ncols = ncols;
+        M.nnz = nnz;
+        int rc = M.xadj.alloc((size_t)nrows + 1);
+        if (rc == CEL_OK) rc = M.cols.alloc(nnz > 0 ? nnz : 1);
+        if (rc == CEL_OK) rc = M.vals.alloc(nnz > 0 ? nnz : 1);
+        if (rc < 0) { delete h; return rc; }
+        (void)hipMemcpyAsync(M.xadj.p, xadj, sizeof(int) * ((size_t)nrows + 1), hipMemcpyDeviceToDevice, rt().stream);
+        if (nnz) {
+            (void)hipMemcpyAsync(M.cols.p, cols, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToDevice, rt().stream);
+            (void)hipMemcpyAsync(M.vals.p, vals, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToDevice, rt().stream);
+        }
+        rc = launch_extract_diag(M);
+        if (rc < 0) { delete h; return rc; }
+        M.has_diag = true;
+        *A = h;
+        return CEL_OK;
+    }
+    // host arrays: validate like the reference's consumers assume
+    if (xadj[0] != index_base || xadj[nrows] - index_base != nnz) {
+        delete h;
+        return fail(CEL_ERR_BAD_MATRIX, "xadj does not start at index_base or end at nnz");
+    }
+    std::vector<int> x0((size_t)nrows + 1), c0((size_t)nnz);
+    for (int i = 0; i <= nrows; ++i) {
+        x0[i] = xadj[i] - index_base;
+        if (i > 0 && x0[i] < x0[i - 1]) {
+            delete h;
+            return fail(CEL_ERR_BAD_MATRIX, "xadj is not non-decreasing at row " + std::to_string(i));
+        }
+    }
+    for (int j = 0; j < nnz; ++j) {
+        long long c = (long long)cols[j] - index_base - col_offset;
+        if (c < 0 || c >= ncols) {
+            delete h;
+            return fail(CEL_ERR_BAD_MATRIX, "column index out of range at entry " + std::to_string(j));
+        }
+        c0[j] = (int)c;
+    }
+    int rc = mat_from_host(M, nrows, ncols, x0, c0, vals, nnz);
+    if (rc < 0) { delete h; return rc; }
+    *A = h;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_mat_create_coo(cel_gpu_mat** A, int nrows, int ncols, int nnz,
+                                      const int* rows, const int* cols, const double* vals,
+                                      int index_base, int row_offset, int col_offset)
+{
+    CEL_TRY(ensure_init());
+    if (!A || nrows < 0 || nnz < 0 || (nnz > 0 && (!rows || !cols || !vals)))
+        return fail(CEL_ERR_ARG, "bad arguments to cel_gpu_mat_create_coo");
+    // cel_sp_mat_to_csr: xadj from row counts; entries are not moved, so the
+    // COO must already be sorted by row
+    std::vector<int> xadj((size_t)nrows + 1, 0);
+    int prev = -1;
+    for (int k = 0; k < nnz; ++k) {
+        long long r = (long long)rows[k] - index_base - row_offset;
+        if (r < 0 || r >= nrows) return fail(CEL_ERR_BAD_MATRIX, "row index out of range at entry " + std::to_string(k));
+        if (r < prev) return fail(CEL_ERR_BAD_MATRIX, "COO entries are not sorted by row");
+        prev = (int)r;
+        xadj[r + 1] += 1;
+    }
+    xadj[0] = index_base;
+    for (int i = 1; i <= nrows; ++i) xadj[i] += xadj[i - 1];
+    return cel_gpu_mat_create_csr(A, nrows, ncols, nnz, xadj.data(), cols, vals, index_base,
+                                  row_offset, col_offset, CEL_MEM_HOST);
+}
+
+static int gen7(cel_gpu_mat** A, int nx, int ny, int nz, const double* kappa, int memspace)
+{
+    CEL_TRY(ensure_init());
+    if (!A || nx < 1 || ny < 1 || nz < 1) return fail(CEL_ERR_ARG, "bad grid");
+    long long n = (long long)nx * ny * nz;
+    long long nnz = n + 2LL * ((long long)(nx - 1) * ny * nz + (long long)nx * (ny - 1) * nz +
+                               (long long)nx * ny * (nz - 1));
+    if (nnz > 0x7fffffffLL) return fail(CEL_ERR_ARG, "grid too large for 32-bit indices");
+    auto* h = new cel_gpu_mat();
+    Mat& M = h->m;
+    M.n = M.ncols = (int)n;
+    M.nnz = (int)nnz;
+    int rc = M.xadj.alloc((size_t)n + 1);
+    if (rc == CEL_OK) rc = M.cols.alloc((size_t)nnz);
+    if (rc == CEL_OK) rc = M.vals.alloc((size_t)nnz);
+    DevBuf<double> dk;
+    const double* kp = nullptr;
+    if (rc == CEL_OK && kappa) {
+        if (memspace == CEL_MEM_HOST) {
+            rc = upload(dk, kappa, (size_t)n);
+            kp = dk.p;
+        } else {
+            kp = kappa;
+        }
+    }
+    if (rc == CEL_OK) rc = launch_gen_poisson7(nx, ny, nz, kp, M.xadj.p, M.cols.p, M.vals.p);
+    if (rc == CEL_OK && hipStreamSynchronize(rt().stream) != hipSuccess) rc = fail(CEL_ERR_HIP, "generator failed");
+    if (rc == CEL_OK) rc = launch_extract_diag(M);
+    if (rc < 0) { delete h; return rc; }
+    M.has_diag = true;
+    *A = h;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_mat_gen_poisson7(cel_gpu_mat** A, int nx, int ny, int nz)
+{
+    return gen7(A, nx, ny, nz, nullptr, CEL_MEM_HOST);
+}
+
+extern "C" int cel_gpu_mat_gen_diffusion7(cel_gpu_mat** A, int nx, int ny, int nz,
+                                          const double* kappa, int memspace)
+{
+    if (!kappa) return fail(CEL_ERR_ARG, "kappa is null");
+    return gen7(A, nx, ny, nz, kappa, memspace);
+}
+
+extern "C" int cel_gpu_mat_destroy(cel_gpu_mat* A)
+{
+    if (!A) return CEL_OK;
+    if (rt().ready) (void)hipStreamSynchronize(rt().stream);
+    pcg_release_workspace(&A->m);
+    delete A;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_mat_info(const cel_gpu_mat* A, int* nrows, int* ncols, int* nnz,
+                                int* jad_mmax, int* nblocks)
+{
+    if (!A) return fail(CEL_ERR_ARG, "null matrix");
+    if (nrows) *nrows = A->m.n;
+    if (ncols) *ncols = A->m.ncols;
+    if (nnz) *nnz = A->m.nnz;
+    if (jad_mmax) *jad_mmax = A->m.jad_mode >= 0 ? A->m.jad_mmax : -1;
+    if (nblocks) *nblocks = 1 + (int)A->m.halo.size();
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_mat_get_csr(const cel_gpu_mat* A, int* xadj, int* cols, double* vals)
+{
+    if (!A) return fail(CEL_ERR_ARG, "null matrix");
+    const Mat& M = A->m;
+    CEL_TRY(ensure_init());
+    CEL_HIP(hipStreamSynchronize(rt().stream));
+    if (xadj) {
+        CEL_HIP(hipMemcpy(xadj, M.xadj.p, sizeof(int) * ((size_t)M.n + 1), hipMemcpyDeviceToHost));
+        for (int i = 0; i <= M.n; ++i) xadj[i] += 1;
+    }
+    if (cols && M.nnz) {
+        CEL_HIP(hipMemcpy(cols, M.cols.p, sizeof(int) * (size_t)M.nnz, hipMemcpyDeviceToHost));
+        for (int j = 0; j < M.nnz; ++j) cols[j] += 1 + M.col_offset;
+    }
+    if (vals && M.nnz)
+        CEL_HIP(hipMemcpy(vals, M.vals.p, sizeof(double) * (size_t)M.nnz, hipMemcpyDeviceToHost));
+    return CEL_OK;
+}
+
+// ------------------------------------------------------------------ JAD
+extern "C" int cel_gpu_mat_to_jad(cel_gpu_mat* A, int mode)
+{
+    if (!A) return fail(CEL_ERR_ARG, "null matrix");
+    return jad_convert_device(A->m, mode);
+}
+
+extern "C" int cel_gpu_mat_get_jad(const cel_gpu_mat* A, int* perm, int* begin, int* cols,
+                                   double* vals, int* mmax, int* ndefined, long long* nnz_written)
+{
+    if (!A) return fail(CEL_ERR_ARG, "null matrix");
+    const Mat& M = A->m;
+    if (M.jad_mode < 0) return fail(CEL_ERR_STATE, "matrix has not been converted to JAD");
+    CEL_TRY(ensure_init());
+    CEL_HIP(hipStreamSynchronize(rt().stream));
+    if (mmax) *mmax = M.jad_mmax;
+    if (ndefined) *ndefined = M.jad_ndef;
+    if (nnz_written) *nnz_written = M.jad_nnz;
+    if (perm && M.n) {
+        CEL_HIP(hipMemcpy(perm, M.jad_perm.p, sizeof(int) * (size_t)M.n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < M.n; ++i) perm[i] += 1;
+    }
+    if (begin)
+        for (int d = 0; d <= M.jad_mmax; ++d)
+            begin[d] = M.h_jad_begin[d] >= 0 ? M.h_jad_begin[d] + 1 : -1;
+    if (cols && M.jad_nnz) {
+        CEL_HIP(hipMemcpy(cols, M.jad_cols.p, sizeof(int) * (size_t)M.jad_nnz, hipMemcpyDeviceToHost));
+        for (long long j = 0; j < M.jad_nnz; ++j) cols[j] += 1;
+    }
+    if (vals && M.jad_nnz)
+        CEL_HIP(hipMemcpy(vals, M.jad_vals.p, sizeof(double) * (size_t)M.jad_nnz, hipMemcpyDeviceToHost));
+    return CEL_OK;
+}
+
+// ------------------------------------------------------------------ Jacobi / scaling
+extern "C" int cel_gpu_mat_set_jacobi(cel_gpu_mat* A, int row_scaled)
+{
+    if (!A) return fail(CEL_ERR_ARG, "null matrix");
+    CEL_TRY(ensure_init());
+    Mat& M = A->m;
+    if (!M.has_diag) CEL_TRY(launch_extract_diag(M));
+    M.has_diag = true;
+    CEL_TRY(launch_set_jacobi(M, row_scaled));
+    M.has_jac = true;
+    M.jac_row_scaled = row_scaled != 0;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_mat_scale_diagonal(cel_gpu_mat* A)
+{
+    if (!A) return fail(CEL_ERR_ARG, "null matrix");
+    CEL_TRY(ensure_init());
+    Mat& M = A->m;
+    if (M.is_scaled) return fail(CEL_ERR_STATE, "matrix is already scaled");
+    CEL_TRY(launch_scale_rows(M));
+    M.is_scaled = true;
+    // derived copies are stale now
+    M.vals32.release();
+    M.jad_mode = -1;
+    M.jad_vals32.release();
+    CEL_HIP(hipStreamSynchronize(rt().stream));
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_vec_scale_diagonal(const cel_gpu_mat* A, double* b, int memspace)
+{
+    if (!A || !b) return fail(CEL_ERR_ARG, "null argument");
+    CEL_TRY(ensure_init());
+    const Mat& M = A->m;
+    if (memspace == CEL_MEM_DEVICE) {
+        CEL_TRY(launch_scale_vec(M, b));
+        CEL_HIP(hipStreamSynchronize(rt().stream));
+        return CEL_OK;
+    }
+    DevBuf<double> d;
+    CEL_TRY(upload(d, b, (size_t)M.n));
+    CEL_TRY(launch_scale_vec(M, d.p));
+    CEL_HIP(hipMemcpyAsync(b, d.p, sizeof(double) * (size_t)M.n, hipMemcpyDeviceToHost, rt().stream));
+    CEL_HIP(hipStreamSynchronize(rt().stream));
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_mat_get_jacobi(const cel_gpu_mat* A, double* jac)
+{
+    if (!A || !jac) return fail(CEL_ERR_ARG, "null argument");
+    if (!A->m.has_jac) return fail(CEL_ERR_STATE, "Jacobi vector not set");
+    CEL_TRY(ensure_init());
+    CEL_HIP(hipStreamSynchronize(rt().stream));
+    if (A->m.n) CEL_HIP(hipMemcpy(jac, A->m.jac.p, sizeof(double) * (size_t)A->m.n, hipMemcpyDeviceToHost));
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_mat_get_diagonal(const cel_gpu_mat* A, double* diag)
+{
+    if (!A || !diag) return fail(CEL_ERR_ARG, "null argument");
+    if (!A->m.has_diag) return fail(CEL_ERR_STATE, "diagonal not captured");
+    CEL_TRY(ensure_init());
+    CEL_HIP(hipStreamSynchronize(rt().stream));
+    if (A->m.n) CEL_HIP(hipMemcpy(diag, A->m.diag.p, sizeof(double) * (size_t)A->m.n, hipMemcpyDeviceToHost));
+    return CEL_OK;
+}
+
+// ------------------------------------------------------------------ SpMV
+template <typename T>
+static int spmv_impl(Mat& M, int format, const T* x, T* y, int accumulate, int memspace,
+                     const double* ghost, bool distr)
+{
+    CEL_TRY(ensure_init());
+    if (format != CEL_FMT_CSR && format != CEL_FMT_JAD) return fail(CEL_ERR_ARG, "format must be CSR or JAD");
+    if (format == CEL_FMT_JAD && (M.jad_mode < 0 || M.jad_ndef != M.jad_mmax))
+        return fail(CEL_ERR_STATE, "no complete JAD arrays (convert first)");
+    constexpr bool f32 = sizeof(T) == 4;
+    if (f32) {
+        if (format == CEL_FMT_CSR && !M.vals32.p) {
+            CEL_TRY(M.vals32.alloc(M.nnz > 0 ? M.nnz : 1));
+            CEL_TRY(launch_to_f32(M.vals.p, M.vals32.p, M.nnz));
+        }
+        if (format == CEL_FMT_JAD && !M.jad_vals32.p) {
+            CEL_TRY(M.jad_vals32.alloc(M.jad_nnz > 0 ? M.jad_nnz : 1));
+            CEL_TRY(launch_to_f32(M.jad_vals.p, M.jad_vals32.p, M.jad_nnz));
+        }
+    }
+    const T* dx = x;
+    T* dy = y;
+    const double* dg = ghost;
+    DevBuf<T> bx, by;
+    DevBuf<double> bg;
+    hipStream_t s = rt().stream;
+    if (memspace == CEL_MEM_HOST) {
+        CEL_TRY(upload(bx, x, (size_t)M.ncols));
+        CEL_TRY(by.alloc(M.n > 0 ? M.n : 1));
+        if (M.n) CEL_HIP(hipMemcpyAsync(by.p, y, sizeof(T) * M.n, hipMemcpyHostToDevice, s));
+        dx = bx.p;
+        dy = by.p;
+        if (distr && M.ghost_len) {
+            CEL_TRY(upload(bg, ghost, (size_t)M.ghost_len));
+            dg = bg.p;
+        }
+    }
+    if constexpr (f32) {
+        if (format == CEL_FMT_CSR) CEL_TRY(launch_csr_spmv_f32(M, (const float*)dx, (float*)dy, accumulate));
+        else CEL_TRY(launch_jad_spmv_f32(M, (const float*)dx, (float*)dy, accumulate));
+    } else {
+        if (format == CEL_FMT_CSR) CEL_TRY(launch_csr_spmv(M, (const double*)dx, (double*)dy, accumulate));
+        else CEL_TRY(launch_jad_spmv(M, (const double*)dx, (double*)dy, accumulate));
+        if (distr && M.halo_nrows) CEL_TRY(launch_halo_spmv(M, dg, (double*)dy));
+    }
+    if (memspace == CEL_MEM_HOST && M.n)
+        CEL_HIP(hipMemcpyAsync(y, dy, sizeof(T) * M.n, hipMemcpyDeviceToHost, s));
+    CEL_HIP(hipStreamSynchronize(s));
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_spmv(cel_gpu_mat* A, int format, const double* x, double* y, int accumulate,
+                            int memspace)
+{
+    if (!A || (!x && A->m.ncols) || (!y && A->m.n)) return fail(CEL_ERR_ARG, "null argument");
+    return spmv_impl<double>(A->m, format, x, y, accumulate, memspace, nullptr, false);
+}
+
+extern "C" int cel_gpu_spmv_f32(cel_gpu_mat* A, int format, const float* x, float* y,
+                                int accumulate, int memspace)
+{
+    if (!A || (!x && A->m.ncols) || (!y && A->m.n)) return fail(CEL_ERR_ARG, "null argument");
+    return spmv_impl<float>(A->m, format, x, y, accumulate, memspace, nullptr, false);
+}
+
+extern "C" int cel_gpu_spmv_distr(cel_gpu_mat* A, int format, const double* x, const double* ghost,
+                                  double* y, int memspace)
+{
+    if (!A || (!x && A->m.ncols) || (!y && A->m.n) || (!ghost && A->m.ghost_len))
+        return fail(CEL_ERR_ARG, "null argument");
+    return spmv_impl<double>(A->m, format, x, y, 0, memspace, ghost, true);
+}
+
+extern "C" int cel_gpu_spmv_time(cel_gpu_mat* A, int format, int precision, const void* x, void* y,
+                                 int reps, double* mean_ms, double* bytes)
+{
+    if (!A || !x || !y || reps < 1) return fail(CEL_ERR_ARG, "bad arguments to cel_gpu_spmv_time");
+    CEL_TRY(ensure_init());
+    Mat& M = A->m;
+    hipStream_t s = rt().stream;
+    auto once = [&]() -> int {
+        if (precision == CEL_PREC_FP64) {
+            if (format == CEL_FMT_CSR) return launch_csr_spmv(M, (const double*)x, (double*)y, 0);
+            return launch_jad_spmv(M, (const double*)x, (double*)y, 0);
+        }
+        if (format == CEL_FMT_CSR) return launch_csr_spmv_f32(M, (const float*)x, (float*)y, 0);
+        return launch_jad_spmv_f32(M, (const float*)x, (float*)y, 0);
+    };
+    if (format == CEL_FMT_JAD && (M.jad_mode < 0 || M.jad_ndef != M.jad_mmax))
+        return fail(CEL_ERR_STATE, "no complete JAD arrays");
+    if (precision != CEL_PREC_FP64) {
+        if (format == CEL_FMT_CSR && !M.vals32.p) {
+            CEL_TRY(M.vals32.alloc(M.nnz > 0 ? M.nnz : 1));
+            CEL_TRY(launch_to_f32(M.vals.p, M.vals32.p, M.nnz));
+        }
+        if (format == CEL_FMT_JAD && !M.jad_vals32.p) {
+            CEL_TRY(M.jad_vals32.alloc(M.jad_nnz > 0 ? M.jad_nnz : 1));
+            CEL_TRY(launch_to_f32(M.jad_vals.p, M.jad_vals32.p, M.jad_nnz));
+        }
+    }
+    CEL_TRY(once());  // warm-up
+    hipEvent_t a, b;
+    CEL_HIP(hipEventCreate(&a));
+    CEL_HIP(hipEventCreate(&b));
+    CEL_HIP(hipEventRecord(a, s));
+    for (int i = 0; i < reps; ++i) CEL_TRY(once());
+    CEL_HIP(hipEventRecord(b, s));
+    CEL_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    CEL_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (mean_ms) *mean_ms = ms / reps;
+    if (bytes) *bytes = spmv_bytes(M, format, precision);
+    return CEL_OK;
+}
+
+// ------------------------------------------------------------------ distributed split
+extern "C" int cel_gpu_mat_distr_split(cel_gpu_mat** A, int nranks, const int* vtxdist, int my_rank,
+                                       int nnz, const int* rows, const int* cols,
+                                       const double* vals, int index_base)
+{
+    CEL_TRY(ensure_init());
+    if (!A || nranks < 1 || !vtxdist || my_rank < 0 || my_rank >= nranks || nnz < 0 ||
+        (nnz > 0 && (!rows || !cols || !vals)) || (index_base != 0 && index_base != 1))
+        return fail(CEL_ERR_ARG, "bad arguments to cel_gpu_mat_distr_split");
+    std::vector<int> vd(vtxdist, vtxdist + nranks + 1);
+    for (int p = 0; p < nranks; ++p)
+        if (vd[p + 1] < vd[p]) return fail(CEL_ERR_ARG, "vtxdist is not non-decreasing");
+    const int n = vd[my_rank + 1] - vd[my_rank];
+    const int my_off = vd[my_rank] - 1;
+    // owner of each entry's column (cel_sp_mat_distr_get_proc_id: vtxdist(p+1) <= col < vtxdist(p+2))
+    std::vector<int> owner((size_t)nnz), per_rank(nranks, 0);
+    int prev_row = -1;
+    for (int k = 0; k < nnz; ++k) {
+        const int gr = rows[k] - index_base + 1, gc = cols[k] - index_base + 1;
+        if (gr - my_off < 1 || gr - my_off > n)
+            return fail(CEL_ERR_BAD_MATRIX, "row " + std::to_string(gr) + " is not owned by this rank");
+        if (gr < prev_row) return fail(CEL_ERR_BAD_MATRIX, "COO entries are not sorted by row");
+        prev_row = gr;
+        auto it = std::upper_bound(vd.begin(), vd.end(), gc);
+        int p = (int)(it - vd.begin()) - 1;
+        if (p < 0 || p >= nranks || gc >= vd[p + 1])
+            return fail(CEL_ERR_BAD_MATRIX, "column index out of range at entry " + std::to_string(k));
+        owner[k] = p;
+        per_rank[p] += 1;
+    }
+    // block order: ascending ranks with entries, my rank moved to the front (:117-131)
+    std::vector<int> procs;
+    for (int p = 0; p < nranks; ++p)
+        if (per_rank[p] > 0) procs.push_back(p);
+    if (std::find(procs.begin(), procs.end(), my_rank) == procs.end())
+        return fail(CEL_ERR_BAD_MATRIX, "rank has no entries in its own column range");
+    if (procs.size() > 1 && procs[0] != my_rank) {
+        procs.erase(std::find(procs.begin(), procs.end(), my_rank));
+        procs.insert(procs.begin(), my_rank);
+    }
+    std::vector<int> block_of(nranks, -1);
+    for (size_t b = 0; b < procs.size(); ++b) block_of[procs[b]] = (int)b;
+
+    auto* h = new cel_gpu_mat();
+    Mat& M = h->m;
+    M.nranks = nranks;
+    M.my_rank = my_rank;
+    M.vtxdist = vd;
+    M.row_offset = my_off;
+    M.col_offset = my_off;
+    M.h_diag_split.assign((size_t)n, 0.0);
+    const int nb = (int)procs.size();
+    M.halo.resize(nb - 1);
+    for (int b = 1; b < nb; ++b) {
+        HaloBlock& H = M.halo[b - 1];
+        H.proc = procs[b];
+        H.col_offset = vd[procs[b]] - 1;
+        H.nnz = per_rank[procs[b]];
+        H.h_rows.reserve(H.nnz);
+        H.h_cols.reserve(H.nnz);
+        H.h_vals.reserve(H.nnz);
+    }
+    M.h_local_rows.reserve(per_rank[my_rank]);
+    M.h_local_cols.reserve(per_rank[my_rank]);
+    M.h_local_vals.reserve(per_rank[my_rank]);
+    for (int k = 0; k < nnz; ++k) {
+        const int gr = rows[k] - index_base + 1, gc = cols[k] - index_base + 1;
+        const int b = block_of[owner[k]];
+        if (gc == gr) M.h_diag_split[gr - my_off - 1] = vals[k];
+        if (b == 0) {
+            M.h_local_rows.push_back(gr);
+            M.h_local_cols.push_back(gc);
+            M.h_local_vals.push_back(vals[k]);
+        } else {
+            HaloBlock& H = M.halo[b - 1];
+            H.h_rows.push_back(gr);
+            H.h_cols.push_back(gc);
+            H.h_vals.push_back(vals[k]);
+        }
+    }
+    // local block -> CSR (cel_sp_mat_to_csr), 0-based local columns on the device
+    const int nl = (int)M.h_local_rows.size();
+    std::vector<int> xl((size_t)n + 1, 0), cl((size_t)nl);
+    for (int k = 0; k < nl; ++k) {
+        xl[M.h_local_rows[k] - my_off] += 1;
+        cl[k] = M.h_local_cols[k] - my_off - 1;
+    }
+    for (int i = 1; i <= n; ++i) xl[i] += xl[i - 1];
+    int rc = mat_from_host(M, n, n, xl, cl, M.h_local_vals.data(), nl);
+    if (rc < 0) { delete h; return rc; }
+    // the reference keeps the diagonal captured during the split
+    rc = upload(M.diag, M.h_diag_split.data(), (size_t)n);
+    if (rc < 0) { delete h; return rc; }
+    // halo blocks: receive lists (ascending referenced columns) and ghost slots
+    int ghost = 0;
+    std::vector<std::vector<int>> slot(nb);
+    for (int b = 1; b < nb; ++b) {
+        HaloBlock& H = M.halo[b - 1];
+        const int plen = vd[H.proc + 1] - vd[H.proc];
+        std::vector<int> pos((size_t)plen, -1);
+        for (int c : H.h_cols) pos[c - H.col_offset - 1] = 0;
+        H.h_recv_perm.clear();
+        for (int j = 0; j < plen; ++j)
+            if (pos[j] == 0) {
+                pos[j] = (int)H.h_recv_perm.size();
+                H.h_recv_perm.push_back(j + 1);
+            }
+        H.recv_len = (int)H.h_recv_perm.size();
+        H.recv_base = ghost;
+        ghost += H.recv_len;
+        H.h_xadj.assign((size_t)n + 1, 0);
+        H.h_xadj[0] = 1;
+        std::vector<int> cnt((size_t)n, 0);
+        for (int r : H.h_rows) cnt[r - my_off - 1] += 1;
+        for (int i = 1; i <= n; ++i) H.h_xadj[i] = H.h_xadj[i - 1] + cnt[i - 1];
+        slot[b].resize(H.h_cols.size());
+        for (size_t k = 0; k < H.h_cols.size(); ++k)
+            slot[b][k] = H.recv_base + pos[H.h_cols[k] - H.col_offset - 1];
+    }
+    M.ghost_len = ghost;
+    // merged halo entries per local row: block order, then entry order
+    std::vector<int> rowcnt((size_t)n, 0);
+    for (int b = 1; b < nb; ++b)
+        for (int r : M.halo[b - 1].h_rows) rowcnt[r - my_off - 1] += 1;
+    std::vector<int> hrows, hptr(1, 0);
+    std::vector<int> rowq((size_t)n, -1);
+    for (int i = 0; i < n; ++i)
+        if (rowcnt[i]) {
+            rowq[i] = (int)hrows.size();
+            hrows.push_back(i);
+            hptr.push_back(hptr.back() + rowcnt[i]);
+        }
+    std::vector<int> fillp(hptr.begin(), hptr.end() - 1);
+    std::vector<int> hcols((size_t)hptr.back());
+    std::vector<double> hvals((size_t)hptr.back());
+    for (int b = 1; b < nb; ++b) {
+        const HaloBlock& H = M.halo[b - 1];
+        for (size_t k = 0; k < H.h_rows.size(); ++k) {
+            const int q = rowq[H.h_rows[k] - my_off - 1];
+            const int at = fillp[q]++;
+            hcols[at] = slot[b][k];
+            hvals[at] = H.h_vals[k];
+        }
+    }
+    M.halo_nrows = (int)hrows.size();
+    rc = upload(M.halo_rows, hrows.data(), hrows.size());
+    if (rc == CEL_OK) rc = upload(M.halo_ptr, hptr.data(), hptr.size());
+    if (rc == CEL_OK) rc = upload(M.halo_cols, hcols.data(), hcols.size());
+    if (rc == CEL_OK) rc = upload(M.halo_vals, hvals.data(), hvals.size());
+    if (rc == CEL_OK && hipStreamSynchronize(rt().stream) != hipSuccess) rc = fail(CEL_ERR_HIP, "upload failed");
+    if (rc < 0) { delete h; return rc; }
+    *A = h;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_mat_get_block(const cel_gpu_mat* A, int b, int* proc, int* nnz, int* rows,
+                                     int* cols, double* vals, int* xadj, int* recv_len,
+                                     int* recv_perm)
+{
+    if (!A) return fail(CEL_ERR_ARG, "null matrix");
+    const Mat& M = A->m;
+    if (b < 0 || b > (int)M.halo.size()) return fail(CEL_ERR_ARG, "block index out of range");
+    if (b == 0) {
+        if (proc) *proc = M.my_rank;
+        const int k = M.h_local_rows.empty() ? M.nnz : (int)M.h_local_rows.size();
+        if (nnz) *nnz = k;
+        if (!M.h_local_rows.empty()) {
+            if (rows) std::copy(M.h_local_rows.begin(), M.h_local_rows.end(), rows);
+            if (cols) std::copy(M.h_local_cols.begin(), M.h_local_cols.end(), cols);
+            if (vals) std::copy(M.h_local_vals.begin(), M.h_local_vals.end(), vals);
+        }
+        if (xadj) CEL_TRY(cel_gpu_mat_get_csr(A, xadj, nullptr, nullptr));
+        if (recv_len) *recv_len = 0;
+        return CEL_OK;
+    }
+    const HaloBlock& H = M.halo[b - 1];
+    if (proc) *proc = H.proc;
+    if (nnz) *nnz = H.nnz;
+    if (rows) std::copy(H.h_rows.begin(), H.h_rows.end(), rows);
+    if (cols) std::copy(H.h_cols.begin(), H.h_cols.end(), cols);
+    if (vals) std::copy(H.h_vals.begin(), H.h_vals.end(), vals);
+    if (xadj) std::copy(H.h_xadj.begin(), H.h_xadj.end(), xadj);
+    if (recv_len) *recv_len = H.recv_len;
+    if (recv_perm) std::copy(H.h_recv_perm.begin(), H.h_recv_perm.end(), recv_perm);
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_mat_set_send_list(cel_gpu_mat* A, int peer, int len, const int* perm)
+{
+    if (!A || len < 0 || (len > 0 && !perm)) return fail(CEL_ERR_ARG, "bad send list");
+    CEL_TRY(ensure_init());
+    Mat& M = A->m;
+    for (int j = 0; j < len; ++j)
+        if (perm[j] < 1 || perm[j] > M.n) return fail(CEL_ERR_BAD_MATRIX, "send index out of range");
+    // keep lists ordered by peer; replace an existing list for the same peer
+    std::vector<std::vector<int>> lists;
+    std::vector<int> peers;
+    {
+        std::vector<int> all((size_t)M.send_total);
+        if (M.send_total)
+            CEL_HIP(hipMemcpy(all.data(), M.send_perm.p, sizeof(int) * M.send_total, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < M.send_proc.size(); ++q) {
+            if (M.send_proc[q] == peer) continue;
+            peers.push_back(M.send_proc[q]);
+            lists.emplace_back(all.begin() + M.send_base[q], all.begin() + M.send_base[q] + M.send_len[q]);
+        }
+    }
+    std::vector<int> mine((size_t)len);
+    for (int j = 0; j < len; ++j) mine[j] = perm[j] - 1;
+    auto at = std::lower_bound(peers.begin(), peers.end(), peer) - peers.begin();
+    peers.insert(peers.begin() + at, peer);
+    lists.insert(lists.begin() + at, mine);
+    M.send_proc = peers;
+    M.send_len.clear();
+    M.send_base.clear();
+    std::vector<int> flat;
+    for (auto& l : lists) {
+        M.send_base.push_back((int)flat.size());
+        M.send_len.push_back((int)l.size());
+        flat.insert(flat.end(), l.begin(), l.end());
+    }
+    M.send_total = (int)flat.size();
+    CEL_TRY(upload(M.send_perm, flat.data(), flat.size()));
+    CEL_HIP(hipStreamSynchronize(rt().stream));
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_face_pack(cel_gpu_mat* A, const double* x, double* sendbuf, int memspace)
+{
+    if (!A || (!x && A->m.n) || (!sendbuf && A->m.send_total)) return fail(CEL_ERR_ARG, "null argument");
+    CEL_TRY(ensure_init());
+    Mat& M = A->m;
+    hipStream_t s = rt().stream;
+    if (memspace == CEL_MEM_DEVICE) {
+        CEL_TRY(launch_face_pack(M, x, sendbuf));
+        CEL_HIP(hipStreamSynchronize(s));
+        return CEL_OK;
+    }
+    DevBuf<double> dx, db;
+    CEL_TRY(upload(dx, x, (size_t)M.n));
+    CEL_TRY(db.alloc(M.send_total > 0 ? M.send_total : 1));
+    CEL_TRY(launch_face_pack(M, dx.p, db.p));
+    if (M.send_total)
+        CEL_HIP(hipMemcpyAsync(sendbuf, db.p, sizeof(double) * M.send_total, hipMemcpyDeviceToHost, s));
+    CEL_HIP(hipStreamSynchronize(s));
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_mat_ghost_len(const cel_gpu_mat* A, int* ghost_len, int* send_total)
+{
+    if (!A) return fail(CEL_ERR_ARG, "null matrix");
+    if (ghost_len) *ghost_len = A->m.ghost_len;
+    if (send_total) *send_total = A->m.send_total;
+    return CEL_OK;
+}
+
+// ------------------------------------------------------------------ PCG
+extern "C" void cel_gpu_pcg_params_default(cel_pcg_params* p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->tol = 1e-8;           // eps of the solver API (cel_elmsolver_module.f90:141)
+    p->tol_mode = CEL_TOL_ABS;
+    p->iter_max = 1000;      // cel_input_parameters_module.f90:154
+    p->x0_mode = CEL_X0_ONE; // cel_cgalg_module.f90:209
+    p->precision = CEL_PREC_FP64;
+    p->format = CEL_FMT_CSR;
+    p->memspace = CEL_MEM_HOST;
+    p->use_halo = 0;
+    p->ru_delta = 0.1;
+    p->ru_restart = 0;
+    p->check_every = 16;
+    p->use_graph = 0;
+    p->profile = 0;
+}
+
+extern "C" int cel_gpu_pcg_solve(cel_gpu_mat* A, const double* b, double* x,
+                                 const cel_pcg_params* params, cel_pcg_stats* stats, double* hist)
+{
+    if (!A || !b || !x) return fail(CEL_ERR_ARG, "null argument");
+    cel_pcg_params p;
+    if (params) p = *params;
+    else cel_gpu_pcg_params_default(&p);
+    cel_pcg_stats st;
+    std::memset(&st, 0, sizeof(st));
+    int rc = pcg_solve(A->m, b, x, p, st, hist);
+    if (stats) *stats = st;
+    return rc;
+}
+
+// ------------------------------------------------------------------ adjoint
+extern "C" int cel_gpu_cg_adjoint(cel_gpu_mat* A, const double* x, const double* dLdx, double* dLdb,
+                                  double* dLda, const cel_pcg_params* params, cel_pcg_stats* stats)
+{
+    if (!A || !x || !dLdx || !dLdb) return fail(CEL_ERR_ARG, "null argument");
+    CEL_TRY(ensure_init());
+    Mat& M = A->m;
+    cel_pcg_params p;
+    if (params) p = *params;
+    else {
+        cel_gpu_pcg_params_default(&p);
+        p.x0_mode = CEL_X0_ZERO;
+    }
+    // A is symmetric for every matrix this path generates; A^T lambda = dLdx
+    // is solved with the same PCG
+    cel_pcg_stats st;
+    std::memset(&st, 0, sizeof(st));
+    int rc = pcg_solve(M, dLdx, dLdb, p, st, nullptr);
+    if (stats) *stats = st;
+    if (rc < 0) return rc;
+    if (dLda) {
+        hipStream_t s = rt().stream;
+        if (p.memspace == CEL_MEM_DEVICE) {
+            CEL_TRY(launch_sddmm(M, dLdb, x, dLda));
+        } else {
+            DevBuf<double> dl, dx, dg;
+            CEL_TRY(upload(dl, dLdb, (size_t)M.n));
+            CEL_TRY(upload(dx, x, (size_t)M.ncols));
+            CEL_TRY(dg.alloc(M.nnz > 0 ? M.nnz : 1));
+            CEL_TRY(launch_sddmm(M, dl.p, dx.p, dg.p));
+            if (M.nnz) CEL_HIP(hipMemcpyAsync(dLda, dg.p, sizeof(double) * M.nnz, hipMemcpyDeviceToHost, s));
+            CEL_HIP(hipStreamSynchronize(s));
+        }
+        CEL_HIP(hipStreamSynchronize(s));
+    }
+    return rc;
+}
+
+extern "C" int cel_gpu_diffusion7_kappa_grad(int nx, int ny, int nz, const double* kappa,
+                                             const double* x, const double* lambda, double* grad,
+                                             int memspace)
+{
+    if (!kappa || !x || !lambda || !grad || nx < 1 || ny < 1 || nz < 1)
+        return fail(CEL_ERR_ARG, "bad arguments");
+    CEL_TRY(ensure_init());
+    const size_t n = (size_t)nx * ny * nz;
+    hipStream_t s = rt().stream;
+    if (memspace == CEL_MEM_DEVICE) {
+        CEL_TRY(launch_kappa_grad(nx, ny, nz, kappa, x, lambda, grad));
+        CEL_HIP(hipStreamSynchronize(s));
+        return CEL_OK;
+    }
+    DevBuf<double> dk, dx, dl, dg;
+    CEL_TRY(upload(dk, kappa, n));
+    CEL_TRY(upload(dx, x, n));
+    CEL_TRY(upload(dl, lambda, n));
+    CEL_TRY(dg.alloc(n));
+    CEL_TRY(launch_kappa_grad(nx, ny, nz, dk.p, dx.p, dl.p, dg.p));
+    CEL_HIP(hipMemcpyAsync(grad, dg.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    CEL_HIP(hipStreamSynchronize(s));
+    return CEL_OK;
+}

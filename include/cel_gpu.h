@@ -1,0 +1,263 @@
+/*
+ * cel_gpu.h -- C ABI of the MI355X (gfx950) Jacobi-PCG library.
+ *
+ * Drop-in boundary for the hot path of the CEL linear solver.  The
+ * reference exposes this path only as Fortran module procedures; every
+ * entry point below names the reference interface it replaces
+ * (paths relative to the reference tree).  Fortran reaches these
+ * functions through ISO_C_BINDING (cel_v3_amd/fortran/cel_gpu_mod.f90).
+ *
+ * Conventions
+ *   - INDEX = int (32-bit), REAL = double, as src/include/base/cel_types.h:12-13.
+ *   - Index arrays take an explicit index_base (0 or 1).  Arrays handed back
+ *     for parity checks are 1-based, exactly as the reference stores them.
+ *   - memspace says whether a vector pointer is host (CEL_MEM_HOST) or
+ *     device memory (CEL_MEM_DEVICE, from cel_gpu_malloc).
+ *   - Every function returns a status code; nothing aborts the process
+ *     (the reference STOPs in cel_error, src/fbase/cel_error_module.f90:17-32).
+ *     cel_gpu_last_error() returns the message of the last failure.
+ *   - One host thread per matrix handle.  All work is ordered on the
+ *     library's stream; calls return after their results are complete.
+ */
+#ifndef CEL_GPU_H
+#define CEL_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define CEL_OK 0
+#define CEL_NOT_CONVERGED 1     /* iter_max reached; results and stats valid */
+#define CEL_ERR_ARG (-1)
+#define CEL_ERR_HIP (-2)
+#define CEL_ERR_NO_DEVICE (-3)
+#define CEL_ERR_BAD_MATRIX (-4) /* index out of range, unsorted rows, bad xadj */
+#define CEL_ERR_BAD_DIAG (-5)   /* zero or missing diagonal entry */
+#define CEL_ERR_BREAKDOWN (-6)  /* p.Ap <= 0 or not finite */
+#define CEL_ERR_NONFINITE (-7)  /* NaN/Inf in the right-hand side */
+#define CEL_ERR_OOM (-8)
+#define CEL_ERR_REF_UNDEFINED (-9) /* the reference's result is undefined here */
+#define CEL_ERR_STATE (-10)     /* call order (e.g. JAD requested before conversion) */
+
+/* ---- enumerations ------------------------------------------------------ */
+/* storage formats; values equal MV_CSR/MV_JAD/MV_COO of
+ * src/fbase/cel_types_module.f90:57-59 */
+#define CEL_FMT_CSR 1
+#define CEL_FMT_JAD 2
+#define CEL_FMT_COO 3
+
+#define CEL_MEM_HOST 0
+#define CEL_MEM_DEVICE 1
+
+/* initial guess: the reference hard-codes x0 = 1 (src/fcgalg/cel_cgalg_module.f90:209) */
+#define CEL_X0_ONE 0
+#define CEL_X0_ZERO 1
+#define CEL_X0_USER 2
+
+/* stopping quantity: the reference stops on the absolute 2-norm,
+ * norm2 > eps (cel_cgalg_module.f90:247-249) */
+#define CEL_TOL_ABS 0
+#define CEL_TOL_REL_B 1   /* eps = tol * ||b|| */
+#define CEL_TOL_REL_R0 2  /* eps = tol * ||b - A x0|| */
+
+#define CEL_PREC_FP64 0
+#define CEL_PREC_MIXED 1  /* FP32 inner PCG, FP64 residual/solution (no reference counterpart) */
+
+/* CSR -> JAD diagonal lengths (src/fspmtconverter/cel_sp_mat_jad_converter_module.f90:134-143) */
+#define CEL_JAD_SPARSKIT 0
+#define CEL_JAD_CEL_COMPAT 1
+
+typedef struct cel_gpu_mat cel_gpu_mat;
+
+/* Solver parameters: the fields of cel_cgalg_parameter_type
+ * (src/fcgalg/cel_alg_module.f90:26-40) plus the choices the reference
+ * hard-codes.  cel_gpu_pcg_params_default() fills the reference settings
+ * (x0 = 1, absolute tolerance, CSR, FP64). */
+typedef struct {
+    double tol;        /* eps (absolute) or relative factor, see tol_mode */
+    int tol_mode;      /* CEL_TOL_* */
+    int iter_max;      /* cel_cgalg_parameter_type%iter_max */
+    int x0_mode;       /* CEL_X0_* */
+    int precision;     /* CEL_PREC_* */
+    int format;        /* CEL_FMT_CSR or CEL_FMT_JAD for the local block */
+    int memspace;      /* where b and x live */
+    int use_halo;      /* 1: pack the send faces and add halo blocks every SpMV */
+    double ru_delta;   /* mixed: replace the FP32 residual by the FP64 one when it
+                          has dropped by this factor (default 0.1) */
+    int ru_restart;    /* mixed: 1 also restarts p (classic iterative refinement) */
+    int check_every;   /* iterations per host poll of the device state (default 16) */
+    int use_graph;     /* 1: replay one captured chunk of iterations as a hipGraph */
+    int profile;       /* 1: time every SpMV launch with HIP events (stats) */
+} cel_pcg_params;
+
+/* Per-solve results.  The reference returns iter_done and norm2
+ * (cel_cgalg_module.f90:302-304) and prints the history at E12.6. */
+typedef struct {
+    int status;          /* CEL_OK / CEL_NOT_CONVERGED / error code */
+    int iters;           /* iter_done (inner iterations for mixed) */
+    int outer_iters;     /* mixed: number of FP64 residual replacements */
+    double norm0;        /* ||b - A x0|| */
+    double final_norm;   /* norm2 at exit (FP64 true residual for mixed) */
+    double eps_abs;      /* absolute threshold used */
+    double alpha;        /* final alpha (r.z) */
+    double solve_ms;     /* device time of the iteration loop (HIP events) */
+    double setup_ms;     /* device time of r0/p0 setup */
+    double spmv_ms;      /* profile=1: summed duration of the SpMV launches */
+    int spmv_launches;   /* profile=1: number of timed SpMV launches */
+    double spmv_bytes;   /* algorithmic bytes of one SpMV of the iteration */
+    double iter_bytes;   /* algorithmic bytes of one full iteration */
+} cel_pcg_stats;
+
+/* ---- runtime ----------------------------------------------------------- */
+/* Replaces MPI_Init / cel_omp_init set-up (src/fbase/cel_omp_module.f90:27-91):
+ * selects the GPU and creates the library stream. */
+int cel_gpu_init(int device);
+int cel_gpu_finalize(void);
+const char* cel_gpu_last_error(void);
+int cel_gpu_device_info(char* name, int name_len, int* num_cus);
+
+/* device memory for callers that keep inputs resident in HBM */
+int cel_gpu_malloc(void** ptr, size_t bytes);
+int cel_gpu_free(void* ptr);
+int cel_gpu_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int cel_gpu_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int cel_gpu_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int cel_gpu_synchronize(void);
+
+/* ---- matrices ---------------------------------------------------------- */
+/* CSR local block.  Replaces cel_sp_mat_type (src/fspmt/cel_sp_mat_module.f90:38-71)
+ * holding xadj/cols/values with col_offset subtracted on use
+ * (src/fop/cel_fop_mv_module.f90:219-224).  Arrays are copied; the caller
+ * keeps ownership.  Validates ranges and row order. */
+int cel_gpu_mat_create_csr(cel_gpu_mat** A, int nrows, int ncols, int nnz, const int* xadj,
+                           const int* cols, const double* vals, int index_base, int row_offset,
+                           int col_offset, int memspace);
+
+/* Row-sorted COO.  Replaces cel_sp_mat_to_csr (src/fspmt/cel_sp_mat_module.f90:430-456):
+ * xadj from row counts, entries not moved. */
+int cel_gpu_mat_create_coo(cel_gpu_mat** A, int nrows, int ncols, int nnz, const int* rows,
+                           const int* cols, const double* vals, int index_base, int row_offset,
+                           int col_offset);
+
+/* Device generators of the closed-form test matrices (no reference
+ * counterpart for this matrix: the reference generator is
+ * src/base/cel_discretisation_poisson_3d.c, which scales by 1/h^2 and keeps
+ * Dirichlet identity rows).  poisson7: diag 6, off-diagonal -1, Dirichlet
+ * outside, cell (x,y,z) -> row x + nx*(y + ny*z).  diffusion7: harmonic
+ * mean face coefficients of kappa, boundary faces use kappa_i. */
+int cel_gpu_mat_gen_poisson7(cel_gpu_mat** A, int nx, int ny, int nz);
+int cel_gpu_mat_gen_diffusion7(cel_gpu_mat** A, int nx, int ny, int nz, const double* kappa,
+                               int memspace);
+
+int cel_gpu_mat_destroy(cel_gpu_mat* A);
+int cel_gpu_mat_info(const cel_gpu_mat* A, int* nrows, int* ncols, int* nnz, int* jad_mmax,
+                     int* nblocks);
+/* 1-based host copies of the local block (xadj n+1, cols/vals nnz). */
+int cel_gpu_mat_get_csr(const cel_gpu_mat* A, int* xadj, int* cols, double* vals);
+
+/* ---- CSR -> JAD on the device ------------------------------------------ */
+/* Replaces cel_sp_mat_jad_from_sp_mat_local + cel_sp_mat_jad_dcsort
+ * (src/fspmtconverter/cel_sp_mat_jad_converter_module.f90:57-175).
+ * mode CEL_JAD_SPARSKIT: diagonal d holds #{rows with len >= d} entries.
+ * mode CEL_JAD_CEL_COMPAT: the reference's counts (dcsort leftovers); when
+ * the shortest row is non-empty the reference reads uninitialised counts,
+ * so only the defined leading diagonals are built and CEL_ERR_REF_UNDEFINED
+ * is returned (the JAD SpMV then refuses the matrix). */
+int cel_gpu_mat_to_jad(cel_gpu_mat* A, int mode);
+/* 1-based copies: perm (n), begin (mmax+1, -1 where undefined), cols/vals
+ * (entries written).  *ndefined = diagonals whose data is defined. */
+int cel_gpu_mat_get_jad(const cel_gpu_mat* A, int* perm, int* begin, int* cols, double* vals,
+                        int* mmax, int* ndefined, long long* nnz_written);
+
+/* ---- diagonal, Jacobi vector, row scaling ------------------------------ */
+/* cel_sp_mat_distr_set_jacobi (src/fspmtdistr/cel_sp_mat_distr_format_module.f90:149-203):
+ * jacobi = 1/d, or sqrt|d|/d when row_scaled.  d is the diagonal captured
+ * before any scaling (cel_sp_mat_distr_module.f90:197-207). */
+int cel_gpu_mat_set_jacobi(cel_gpu_mat* A, int row_scaled);
+/* cel_sp_mat_distr_scale (cel_sp_mat_distr_format_module.f90:91-145,276-356):
+ * every local and halo entry of row i divided by sqrt|d_i|. */
+int cel_gpu_mat_scale_diagonal(cel_gpu_mat* A);
+/* cel_sp_mat_distr_boundary_diag_scale (:207-239): b_i = b_i / sqrt|d_i|. */
+int cel_gpu_vec_scale_diagonal(const cel_gpu_mat* A, double* b, int memspace);
+int cel_gpu_mat_get_jacobi(const cel_gpu_mat* A, double* jac);
+int cel_gpu_mat_get_diagonal(const cel_gpu_mat* A, double* diag);
+
+/* ---- SpMV -------------------------------------------------------------- */
+/* cel_fop_mv_axy_distr_worker_local_{csr,jad} (src/fop/cel_fop_mv_module.f90:189-352):
+ * y = A x (accumulate = 0) or y += A x (accumulate = 1), each row summed in
+ * column order from 0.0 and then added to y.  The reference CSR kernel
+ * accumulates; its JAD kernel overwrites (:323-326).  x has ncols entries. */
+int cel_gpu_spmv(cel_gpu_mat* A, int format, const double* x, double* y, int accumulate,
+                 int memspace);
+/* Same in FP32 (FP32 values, x, y and accumulation): the mixed-precision
+ * inner operator.  No reference counterpart. */
+int cel_gpu_spmv_f32(cel_gpu_mat* A, int format, const float* x, float* y, int accumulate,
+                     int memspace);
+/* Times `reps` back-to-back launches of the device SpMV with HIP events on
+ * the library stream (x, y device pointers).  Returns the mean launch time. */
+int cel_gpu_spmv_time(cel_gpu_mat* A, int format, int precision, const void* x, void* y,
+                      int reps, double* mean_ms, double* bytes);
+
+/* ---- distributed block split, face packing, halo SpMV ------------------ */
+/* cel_sp_mat_distr_from_sp_mat_local (src/fspmtdistr/cel_sp_mat_distr_module.f90:34-301):
+ * this rank's row-sorted COO (global ids) is split into the local block
+ * (columns owned by my_rank, becomes CSR) and one halo block per other
+ * owning rank, in ascending rank order, entries kept in input order.
+ * Each halo block's receive list is the ascending list of its referenced
+ * columns (com_buffer_perm, :270-280).  vtxdist has nranks+1 entries with
+ * the reference's 1-based row starts. */
+int cel_gpu_mat_distr_split(cel_gpu_mat** A, int nranks, const int* vtxdist, int my_rank,
+                            int nnz, const int* rows, const int* cols, const double* vals,
+                            int index_base);
+/* Block b (0 = local) of the split, reference layout (global 1-based ids):
+ * proc, entries, rows/cols/vals (NULL to skip), xadj (n+1, NULL to skip),
+ * recv_perm (receive list, blocks >= 1, NULL to skip). */
+int cel_gpu_mat_get_block(const cel_gpu_mat* A, int b, int* proc, int* nnz, int* rows, int* cols,
+                          double* vals, int* xadj, int* recv_len, int* recv_perm);
+/* Send lists: the owner packs, for each peer, the entries that peer listed
+ * (cel_comm_get_comm_buffer_connections, src/fcomm/cel_comm_module.f90:312-357).
+ * perm is 1-based local indices. */
+int cel_gpu_mat_set_send_list(cel_gpu_mat* A, int peer, int len, const int* perm);
+/* Face packing (cel_sp_mat_distr_vec_module.f90:482-486): for every send
+ * list in peer order, sendbuf(j) = x(perm(j)); lists are concatenated. */
+int cel_gpu_face_pack(cel_gpu_mat* A, const double* x, double* sendbuf, int memspace);
+/* Distributed SpMV (cel_fop_mv_axy_distr, src/fop/cel_fop_mv_module.f90:28-87):
+ * y = A_loc x, then for every halo block in order y += A_b ghost_b, where
+ * ghost holds the received values of all halo blocks concatenated in block
+ * order (receive-list order).  The sum order of each row matches the
+ * reference: local row sum first, then halo entries block by block. */
+int cel_gpu_spmv_distr(cel_gpu_mat* A, int format, const double* x, const double* ghost,
+                       double* y, int memspace);
+int cel_gpu_mat_ghost_len(const cel_gpu_mat* A, int* ghost_len, int* send_total);
+
+/* ---- Jacobi-PCG -------------------------------------------------------- */
+void cel_gpu_pcg_params_default(cel_pcg_params* p);
+/* cel_cgalg_jacobi (src/fcgalg/cel_cgalg_module.f90:178-305), same op
+ * order, with x0, tolerance mode and precision selectable.  hist (may be
+ * NULL) receives norm2 after setup and after every iteration
+ * (iter_max+1 doubles); for the mixed form it receives the inner estimate.
+ * Requires cel_gpu_mat_set_jacobi.  With use_halo the halo blocks are
+ * applied with a zero ghost region (one subdomain, no neighbours present)
+ * and the send faces are packed every SpMV. */
+int cel_gpu_pcg_solve(cel_gpu_mat* A, const double* b, double* x, const cel_pcg_params* params,
+                      cel_pcg_stats* stats, double* hist);
+
+/* ---- adjoint (no reference counterpart) -------------------------------- */
+/* For A x = b with L(x): solves A^T lambda = dLdx (A is symmetric here, so
+ * the same PCG is used), returns dLdb = lambda and, when dLda is not NULL,
+ * dL/da_ij = -lambda_i x_j on the CSR pattern of the local block.  x is the
+ * forward solution.  Vectors in memspace. */
+int cel_gpu_cg_adjoint(cel_gpu_mat* A, const double* x, const double* dLdx, double* dLdb,
+                       double* dLda, const cel_pcg_params* params, cel_pcg_stats* stats);
+/* dL/dkappa of the diffusion7 matrix from x and lambda (chain rule through
+ * the harmonic-mean face coefficients). */
+int cel_gpu_diffusion7_kappa_grad(int nx, int ny, int nz, const double* kappa, const double* x,
+                                  const double* lambda, double* grad, int memspace);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

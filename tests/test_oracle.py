@@ -1,0 +1,482 @@
+"""Pins the CPU oracle (oracle/) before it is trusted as the checker.
+
+Each oracle function is checked against something that does not restate its
+own formula: the reference's own generator output (tests/golden, from
+src/base/cel_discretisation_poisson_3d.c), the worked examples of the
+reference converter's behaviour, closed forms, brute force on tiny inputs,
+and independent library routines (numpy dense algebra, scipy's CG).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "ref_generator.npz")
+
+
+def dense(A: O.CSR, ncols=None):
+    n = A.n
+    M = np.zeros((n, ncols or n))
+    for i in range(n):
+        for j in range(A.xadj[i] - 1, A.xadj[i + 1] - 1):
+            M[i, A.cols[j] - 1 - A.col_offset] += A.vals[j]
+    return M
+
+
+def grid_neighbours(nx, ny, nz):
+    """Independent brute-force 7-point matrix (dense) for tiny grids."""
+    n = nx * ny * nz
+    M = np.zeros((n, n))
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                i = x + nx * (y + ny * z)
+                M[i, i] = 6.0
+                for dx, dy, dz in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+                    X, Y, Z = x + dx, y + dy, z + dz
+                    if 0 <= X < nx and 0 <= Y < ny and 0 <= Z < nz:
+                        M[i, X + nx * (Y + ny * Z)] = -1.0
+    return M
+
+
+# ------------------------------------------------------------------ work partition
+def test_work_partition_closed_form():
+    # 32768 rows, 4 threads = 3 workers -> 10920/10920/10928 (thread 0 idle)
+    assert [O.work_distr(32768, 3, w) for w in (1, 2, 3)] == [(1, 10920), (10921, 21840), (21841, 32768)]
+    # fewer than 4*W elements: everything goes to the last worker
+    assert [O.work_distr(7, 3, w) for w in (1, 2, 3)] == [(1, 0), (1, 0), (1, 7)]
+    for L in (0, 1, 5, 99, 1000, 4099):
+        for W in (1, 2, 3, 7):
+            covered = []
+            for w in range(1, W + 1):
+                s, e = O.work_distr(L, W, w)
+                covered.extend(range(s, e + 1))
+                if w < W:
+                    assert (e - s + 1) % 4 == 0
+            assert covered == list(range(1, L + 1))
+
+
+# ------------------------------------------------------------------ generators
+@pytest.mark.parametrize("shape", [(1, 1, 1), (2, 1, 1), (3, 2, 1), (4, 3, 2), (5, 5, 5), (1, 4, 3)])
+def test_poisson7_matches_brute_force(shape):
+    A = O.gen_poisson7(*shape)
+    nx, ny, nz = shape
+    # every boundary face of the box removes one entry from 7 per cell
+    assert A.nnz == O.poisson7_nnz(*shape) == 7 * nx * ny * nz - 2 * (nx * ny + ny * nz + nx * nz)
+    np.testing.assert_array_equal(dense(A), grid_neighbours(*shape))
+    # columns ascending in every row
+    for i in range(A.n):
+        seg = A.cols[A.xadj[i] - 1:A.xadj[i + 1] - 1]
+        assert np.all(np.diff(seg) > 0)
+
+
+def test_poisson7_nnz_baseline_sizes():
+    assert O.poisson7_nnz(32, 32, 32) == 223232
+    assert O.poisson7_nnz(256, 256, 256) == 117047296
+    assert O.poisson7_nnz(384, 384, 384) == 395476992
+
+
+def test_diffusion7_unit_kappa_is_poisson7():
+    A = O.gen_poisson7(4, 3, 5)
+    B = O.gen_diffusion7(4, 3, 5, np.ones(60))
+    np.testing.assert_array_equal(A.xadj, B.xadj)
+    np.testing.assert_array_equal(A.cols, B.cols)
+    np.testing.assert_array_equal(A.vals, B.vals)
+
+
+def test_diffusion7_harmonic_mean_and_symmetry():
+    rng = np.random.default_rng(5)
+    k = np.exp(rng.normal(0, 1, 3 * 4 * 2))
+    M = dense(O.gen_diffusion7(3, 4, 2, k))
+    np.testing.assert_allclose(M, M.T, rtol=0, atol=0)
+    # off-diagonal = -2 ki kj / (ki + kj); diagonal = sum of faces incl. boundary faces = ki
+    i, j = 0, 1
+    assert M[i, j] == -(2 * k[i] * k[j]) / (k[i] + k[j])
+    rowsum = M.sum(axis=1)
+    # A*1 = sum of boundary-face coefficients (each = kappa_i) of the cell
+    cells = [(x, y, z) for z in range(2) for y in range(4) for x in range(3)]
+    for idx, (x, y, z) in enumerate(cells):
+        b = (x == 0) + (x == 2) + (y == 0) + (y == 3) + (z == 0) + (z == 1)
+        assert np.isclose(rowsum[idx], b * k[idx], rtol=1e-13, atol=1e-13)
+
+
+# ------------------------------------------------------------------ reference generator KAT
+def _gold():
+    if not os.path.exists(GOLD):
+        pytest.skip("golden fixtures missing")
+    return np.load(GOLD)
+
+
+def _coo_csr(d, key):
+    rows, cols, vals = d[key + "_rows"], d[key + "_cols"], d[key + "_vals"]
+    meta = d[key + "_meta"]
+    nrows = int(meta[7])
+    row_off = int(rows.min())
+    xadj = O.coo_to_csr(nrows, rows + 1, row_off)
+    return O.CSR(xadj, cols + 1, vals, col_offset=0, row_offset=row_off), row_off
+
+
+@pytest.mark.parametrize("name", ["s7_5x4x3", "s7_6x6x6", "s27_5x5x5"])
+def test_spmv_known_answer_reference_generator(name):
+    """cel_cg.f90:808-837: A x_theory = b, |r_i - b_i| < 1e-7, on the
+    reference generator's own matrix (x_theory = x^2+y^2+z^2).  The 7- and
+    27-point stencils are the ones cel_sp_mat_generate selects
+    (cel_sp_mat_generate_module.f90:73-77); the reference's 19-point stencil
+    does not satisfy the identity on non-cubic grids, so it only serves as
+    an SpMV input elsewhere."""
+    d = _gold()
+    A, _ = _coo_csr(d, name + "_r0")
+    y = O.spmv_csr(A, d[name + "_r0_xt"])
+    assert np.max(np.abs(y - d[name + "_r0_b"])) < 1e-7
+    # the same product by dense algebra
+    np.testing.assert_allclose(dense(A) @ d[name + "_r0_xt"], y, rtol=1e-13, atol=1e-9)
+
+
+def test_coo_to_csr_counts_rows():
+    rows = np.array([1, 1, 3, 3, 3, 4], np.int32)
+    np.testing.assert_array_equal(O.coo_to_csr(4, rows, 0), [1, 3, 3, 6, 7])
+    np.testing.assert_array_equal(O.coo_to_csr(2, rows[2:] - 2, 0), [1, 4, 5])
+
+
+# ------------------------------------------------------------------ SpMV vs brute force
+def random_csr(n, m, density, rng, empty_rows=True, long_row=0):
+    M = (rng.random((n, m)) < density) * rng.normal(size=(n, m))
+    if empty_rows and n > 2:
+        M[1, :] = 0
+    if long_row:
+        M[0, : min(long_row, m)] = rng.normal(size=min(long_row, m))
+    xadj = [1]
+    cols, vals = [], []
+    for i in range(n):
+        nzr = np.nonzero(M[i])[0]
+        cols.extend(nzr + 1)
+        vals.extend(M[i, nzr])
+        xadj.append(xadj[-1] + len(nzr))
+    return O.CSR(np.array(xadj), np.array(cols), np.array(vals)), M
+
+
+@pytest.mark.parametrize("workers", [1, 3])
+def test_spmv_csr_brute_force(workers):
+    rng = np.random.default_rng(1)
+    A, M = random_csr(70, 50, 0.1, rng)
+    x = rng.normal(size=50)
+    np.testing.assert_allclose(O.spmv_csr(A, x, workers=workers), M @ x, rtol=1e-13, atol=1e-13)
+    y0 = rng.normal(size=70)
+    np.testing.assert_allclose(O.spmv_csr(A, x, y0), y0 + M @ x, rtol=1e-13, atol=1e-13)
+
+
+def test_spmv_csr_row_order_is_sequential():
+    # a row whose sum depends on the order: ((1e16 + 1) - 1e16) = 0 in FP64
+    A = O.CSR([1, 4], [1, 2, 3], [1e16, 1.0, -1e16])
+    assert O.spmv_csr(A, np.ones(3))[0] == 0.0
+    A2 = O.CSR([1, 4], [1, 3, 2], [1e16, -1e16, 1.0])
+    assert O.spmv_csr(A2, np.ones(3))[0] == 1.0
+
+
+def test_spmv_f32_brute_force():
+    rng = np.random.default_rng(2)
+    A, M = random_csr(40, 40, 0.15, rng)
+    x = rng.normal(size=40).astype(np.float32)
+    y = O.spmv_csr_f32(A, x)
+    np.testing.assert_allclose(y, (M.astype(np.float32) @ x), rtol=2e-5, atol=2e-5)
+
+
+def test_spmv_coo_brute_force():
+    rng = np.random.default_rng(3)
+    A, M = random_csr(30, 30, 0.2, rng)
+    rows = np.repeat(np.arange(1, 31), np.diff(A.xadj))
+    x = rng.normal(size=30)
+    y = O.spmv_coo(rows, A.cols, A.vals, 0, 0, x, np.zeros(30))
+    np.testing.assert_allclose(y, M @ x, rtol=1e-13, atol=1e-13)
+
+
+# ------------------------------------------------------------------ JAD conversion
+def jad_from_lengths(lens):
+    xadj = np.concatenate([[1], 1 + np.cumsum(lens)]).astype(np.int32)
+    nnz = int(sum(lens))
+    cols = np.concatenate([np.arange(1, L + 1) for L in lens]) if nnz else np.zeros(0)
+    vals = np.arange(1, nnz + 1, dtype=np.float64)
+    return O.CSR(xadj, cols, vals, col_offset=0)
+
+
+def test_dcsort_is_stable_descending():
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        lens = rng.integers(2, 9, size=rng.integers(1, 40))
+        icnt, perm = O.jad_dcsort(lens, int(lens.min()), int(lens.max()))
+        expect = np.argsort(-lens, kind="stable") + 1
+        np.testing.assert_array_equal(perm, expect)
+        for v in range(lens.min(), lens.max() + 1):
+            assert icnt[v - lens.min()] == np.sum(lens > v)
+
+
+def test_jad_worked_examples():
+    # ilo = 0: reference counts equal SPARSKIT's: lengths [0,2,1] -> [2,1]
+    for mode in (O.JAD_SPARSKIT, O.JAD_CEL_COMPAT):
+        j = O.jad_convert(jad_from_lengths([0, 2, 1]), mode)
+        np.testing.assert_array_equal(np.diff(j["begin"]), [2, 1])
+        np.testing.assert_array_equal(j["perm"], [2, 3, 1])
+    # ilo >= 1: lengths [2,3,1,3] -> reference [3,2,0] vs correct [4,3,2]
+    s = O.jad_convert(jad_from_lengths([2, 3, 1, 3]), O.JAD_SPARSKIT)
+    c = O.jad_convert(jad_from_lengths([2, 3, 1, 3]), O.JAD_CEL_COMPAT)
+    np.testing.assert_array_equal(np.diff(s["begin"]), [4, 3, 2])
+    np.testing.assert_array_equal(s["perm"], [2, 4, 1, 3])
+    np.testing.assert_array_equal(c["perm"], [2, 4, 1, 3])
+    assert c["ndefined"] == 3
+    np.testing.assert_array_equal(np.diff(c["begin"][:4]), [3, 2, 0])
+
+
+def test_jad_4cube_reference_counts():
+    A = O.gen_poisson7(4, 4, 4)
+    c = O.jad_convert(A, O.JAD_CEL_COMPAT)
+    s = O.jad_convert(A, O.JAD_SPARSKIT)
+    assert c["ilo"] == 4 and c["mmax"] == 7 and c["ndefined"] == 4
+    np.testing.assert_array_equal(np.diff(c["begin"][:5]), [56, 32, 8, 0])
+    assert list(c["begin"][5:]) == [-1, -1, -1]
+    np.testing.assert_array_equal(np.diff(s["begin"]), [64, 64, 64, 64, 56, 32, 8])
+    np.testing.assert_array_equal(s["perm"], c["perm"])
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4), (5, 3, 2), (7, 1, 1)])
+def test_jad_spmv_equals_csr_bitwise(shape):
+    A = O.gen_poisson7(*shape)
+    j = O.jad_convert(A)
+    x = np.random.default_rng(6).random(A.n)
+    np.testing.assert_array_equal(O.spmv_jad(j, x), O.spmv_csr(A, x))
+    # perm = stable descending sort of row lengths
+    np.testing.assert_array_equal(j["perm"], np.argsort(-np.diff(A.xadj), kind="stable") + 1)
+
+
+def test_jad_random_rows_with_empty_rows():
+    rng = np.random.default_rng(7)
+    A, M = random_csr(50, 50, 0.1, rng)
+    for mode in (O.JAD_SPARSKIT, O.JAD_CEL_COMPAT):  # ilo == 0 here: modes agree
+        j = O.jad_convert(A, mode)
+        assert j["ilo"] == 0 and j["ndefined"] == j["mmax"]
+        np.testing.assert_allclose(O.spmv_jad(j, np.ones(50)), M.sum(axis=1), rtol=1e-13, atol=1e-13)
+
+
+# ------------------------------------------------------------------ Jacobi / scaling
+def test_jacobi_and_scaling_closed_forms():
+    d = np.array([4.0, -9.0, 0.25])
+    np.testing.assert_array_equal(O.set_jacobi(d), 1.0 / d)
+    np.testing.assert_array_equal(O.set_jacobi(d, scaled=True), np.sqrt(np.abs(d)) / d)
+    A = O.CSR([1, 3, 4, 5], [1, 2, 2, 3], [4.0, 2.0, -9.0, 0.25])
+    S = O.scale_csr(A, d)
+    np.testing.assert_array_equal(S.vals, [2.0, 1.0, -3.0, 0.5])
+    # scaled diagonal = sign(d) sqrt|d| and jacobi(scaled) = 1 / scaled diagonal
+    np.testing.assert_allclose(1.0 / O.set_jacobi(d, scaled=True), np.sign(d) * np.sqrt(np.abs(d)))
+    np.testing.assert_array_equal(O.scale_vec([4.0, 9.0, 1.0], d), [2.0, 3.0, 2.0])
+
+
+# ------------------------------------------------------------------ PCG
+def textbook_pcg(M, jac, b, x0, eps, iter_max):
+    """Dense numpy Jacobi-PCG in the reference's op order, written from the
+    textbook recurrences (independent of the C restatement)."""
+    x = x0.copy()
+    r = b - M @ x
+    z = jac * r
+    p = z.copy()
+    a = r @ z
+    nrm = [np.sqrt(r @ r)]
+    k = 0
+    while k < iter_max and abs(a) > 1.1 * np.finfo(float).eps and nrm[-1] > eps:
+        v = M @ p
+        k += 1
+        s = a / (v @ p)
+        x = x + s * p
+        r = r - s * v
+        z = jac * r
+        a_new = r @ z
+        p = (a_new / a) * p + z
+        a = a_new
+        nrm.append(np.sqrt(r @ r))
+    return x, k, np.array(nrm)
+
+
+def test_pcg_matches_textbook_dense():
+    A = O.gen_poisson7(6, 5, 4)
+    rng = np.random.default_rng(0)
+    xt = rng.random(A.n)
+    b = O.spmv_csr(A, xt)
+    jac = O.set_jacobi(O.csr_diagonal(A))
+    eps = 1e-10 * np.linalg.norm(b)
+    r = O.pcg_jacobi(A, jac, b, x0_mode=O.X0_ZERO, eps=eps, iter_max=500, workers=1)
+    xd, kd, hd = textbook_pcg(dense(A), jac, b, np.zeros(A.n), eps, 500)
+    assert abs(r["iters"] - kd) <= 1
+    m = min(len(hd), len(r["hist"]))
+    np.testing.assert_allclose(r["hist"][:m], hd[:m], rtol=1e-9)
+    np.testing.assert_allclose(r["x"], xt, atol=1e-8)
+
+
+def test_pcg_variable_coefficient_vs_scipy_cg():
+    scipy_sparse = pytest.importorskip("scipy.sparse")
+    spla = pytest.importorskip("scipy.sparse.linalg")
+    rng = np.random.default_rng(3)
+    nx, ny, nz = 9, 8, 7
+    k = np.exp(rng.normal(0, 0.5, nx * ny * nz))
+    A = O.gen_diffusion7(nx, ny, nz, k)
+    S = scipy_sparse.csr_matrix((A.vals, A.cols - 1, A.xadj - 1), shape=(A.n, A.n))
+    # scaled so the reference's absolute |alpha| > 1.1 eps stop does not fire
+    # before the norm criterion (see test_pcg_alpha_stop_is_absolute)
+    b = 1e3 * rng.uniform(-1, 1, A.n)
+    d = O.csr_diagonal(A)
+    jac = O.set_jacobi(d)
+    tol = 1e-9
+    r = O.pcg_jacobi(A, jac, b, x0_mode=O.X0_ZERO, eps=tol * np.linalg.norm(b), iter_max=2000, workers=2)
+    count = [0]
+    xs, info = spla.cg(S, b, rtol=tol, atol=0.0, maxiter=2000,
+                       M=scipy_sparse.diags(1.0 / d), callback=lambda _: count.__setitem__(0, count[0] + 1))
+    assert info == 0
+    assert abs(count[0] - r["iters"]) <= 1
+    np.testing.assert_allclose(r["x"], xs, rtol=1e-6, atol=1e-8)
+    # true residual meets the tolerance
+    assert np.linalg.norm(b - S @ r["x"]) <= 1.05 * tol * np.linalg.norm(b)
+
+
+def test_pcg_alpha_stop_is_absolute():
+    """cel_cgalg_module.f90:247-249 also stops when |alpha| = |r.z| <=
+    1.1 eps_mach, an absolute test: a small right-hand side stops early."""
+    A = O.gen_poisson7(6, 6, 6)
+    b = O.spmv_csr(A, np.random.default_rng(0).random(A.n))
+    jac = O.set_jacobi(O.csr_diagonal(A))
+    big = O.pcg_jacobi(A, jac, b, x0_mode=O.X0_ZERO, eps=0.0, iter_max=400)
+    small = O.pcg_jacobi(A, jac, 1e-6 * b, x0_mode=O.X0_ZERO, eps=0.0, iter_max=400)
+    assert small["iters"] < big["iters"]
+    assert abs(small["alpha"][-1]) <= 1.1 * np.finfo(float).eps
+
+
+def test_pcg_x0_one_equals_shifted_zero_start():
+    """The reference hard-codes x0 = 1: solving A x = b + A*1 from x0 = 1
+    follows the same residual sequence as A x = b from x0 = 0."""
+    A = O.gen_poisson7(8, 8, 8)
+    rng = np.random.default_rng(0)
+    b = O.spmv_csr(A, rng.random(A.n))
+    jac = O.set_jacobi(O.csr_diagonal(A))
+    eps = 1e-8 * np.linalg.norm(b)
+    r0 = O.pcg_jacobi(A, jac, b, x0_mode=O.X0_ZERO, eps=eps, iter_max=500)
+    r1 = O.pcg_jacobi(A, jac, b + O.spmv_csr(A, np.ones(A.n)), x0_mode=O.X0_ONE, eps=eps, iter_max=500)
+    assert r0["iters"] == r1["iters"]
+    np.testing.assert_allclose(r1["hist"], r0["hist"], rtol=1e-9)
+    np.testing.assert_allclose(r1["x"] - 1.0, r0["x"], atol=1e-9)
+
+
+def test_pcg_worker_grouping_changes_only_rounding():
+    A = O.gen_poisson7(16, 16, 16)
+    b = O.spmv_csr(A, np.random.default_rng(0).random(A.n))
+    jac = O.set_jacobi(O.csr_diagonal(A))
+    eps = 1e-10 * np.linalg.norm(b)
+    a = O.pcg_jacobi(A, jac, b, x0_mode=O.X0_ZERO, eps=eps, iter_max=1000, workers=1)
+    c = O.pcg_jacobi(A, jac, b, x0_mode=O.X0_ZERO, eps=eps, iter_max=1000, workers=3)
+    assert abs(a["iters"] - c["iters"]) <= 1
+    m = min(a["iters"], c["iters"]) + 1
+    rel = np.abs(a["hist"][:m] - c["hist"][:m]) / a["hist"][:m]
+    assert rel.max() < 1e-9
+
+
+def test_pcg_recursive_residual_tracks_true_residual():
+    A = O.gen_poisson7(10, 9, 8)
+    b = O.spmv_csr(A, np.random.default_rng(1).random(A.n))
+    jac = O.set_jacobi(O.csr_diagonal(A))
+    r = O.pcg_jacobi(A, jac, b, x0_mode=O.X0_ZERO, eps=1e-10 * np.linalg.norm(b), iter_max=1000)
+    true = np.linalg.norm(b - O.spmv_csr(A, r["x"]))
+    assert abs(true - r["norm"]) <= 1e-3 * r["norm"] + 1e-12 * np.linalg.norm(b)
+
+
+def test_pcg_iteration_cap_and_alpha_stop():
+    A = O.gen_poisson7(6, 6, 6)
+    b = O.spmv_csr(A, np.ones(A.n))
+    jac = O.set_jacobi(O.csr_diagonal(A))
+    r = O.pcg_jacobi(A, jac, b, x0_mode=O.X0_ZERO, eps=0.0, iter_max=5)
+    assert r["iters"] == 5
+    # exact start: r0 = 0 -> alpha = 0 -> no iteration
+    r = O.pcg_jacobi(A, jac, b, x0_mode=O.X0_ONE, eps=1e-8, iter_max=5)
+    assert r["iters"] == 0 and r["norm"] == 0.0
+
+
+# ------------------------------------------------------------------ distributed split
+def _decomp_case(d, name, nblocks):
+    """Global row starts and every rank's COO (1-based, global ids)."""
+    parts = []
+    for r in range(nblocks):
+        key = f"{name}_r{r}"
+        parts.append((d[key + "_rows"] + 1, d[key + "_cols"] + 1, d[key + "_vals"], d[key + "_b"]))
+    sizes = [p[3].size for p in parts]
+    vtx = np.concatenate([[1], 1 + np.cumsum(sizes)]).astype(np.int32)
+    return vtx, parts
+
+
+@pytest.mark.parametrize("name,nblocks", [("s7_8x8x4_d2x2x1", 4), ("s27_6x6x6_d2x1x3", 6)])
+def test_distr_split_reassembles_global_spmv(name, nblocks):
+    d = _gold()
+    vtx, parts = _decomp_case(d, name, nblocks)
+    N = int(vtx[-1] - 1)
+    x = np.random.default_rng(2).random(N)
+    # global product from all ranks' rows (dense brute force)
+    G = np.zeros((N, N))
+    for rows, cols, vals, _ in parts:
+        for r_, c_, v_ in zip(rows, cols, vals):
+            G[r_ - 1, c_ - 1] += v_
+    yg = G @ x
+    for rank, (rows, cols, vals, _) in enumerate(parts):
+        s = O.distr_split(vtx, rank, rows, cols, vals)
+        blocks = s["blocks"]
+        assert blocks[0]["proc"] == rank
+        assert [b["proc"] for b in blocks[1:]] == sorted(b["proc"] for b in blocks[1:])
+        assert sum(b["rows"].size for b in blocks) == rows.size
+        off = s["row_offset"]
+        n = s["n"]
+        # local block, then halo blocks through full-length neighbour copies
+        y = np.zeros(n)
+        b0 = blocks[0]
+        y = O.spmv_csr(O.CSR(b0["xadj"], b0["cols"], b0["vals"], b0["col_offset"]),
+                       x[vtx[rank] - 1:vtx[rank + 1] - 1], y)
+        for b in blocks[1:]:
+            p = b["proc"]
+            xn = np.zeros(vtx[p + 1] - vtx[p])
+            perm = b["comm_perm"]
+            assert np.all(np.diff(perm) > 0)
+            # receive buffer = neighbour's packed values, unpacked at perm
+            buf = O.face_pack(perm, x[vtx[p] - 1:vtx[p + 1] - 1])
+            xn[perm - 1] = buf
+            np.testing.assert_array_equal(np.unique(b["cols"] - b["col_offset"]), perm)
+            y = O.spmv_coo(b["rows"], b["cols"], b["vals"], off, b["col_offset"], xn, y)
+        np.testing.assert_allclose(y, yg[vtx[rank] - 1:vtx[rank + 1] - 1], rtol=1e-12, atol=1e-12)
+        # captured diagonal is the block's diagonal
+        np.testing.assert_array_equal(s["diagonal"], np.diag(G)[vtx[rank] - 1:vtx[rank + 1] - 1])
+
+
+# ------------------------------------------------------------------ gradients
+def test_kappa_gradient_central_differences():
+    """Pins the oracle's chain rule: dL/dkappa by central differences of
+    L = 0.5 ||x(kappa) - x_obs||^2 with dense solves on a tiny grid."""
+    rng = np.random.default_rng(4)
+    nx, ny, nz = 3, 3, 2
+    n = nx * ny * nz
+    k = np.exp(rng.normal(0, 1, n))
+    b = rng.random(n)
+    xo = rng.normal(size=n)
+
+    def loss(kk):
+        M = dense(O.gen_diffusion7(nx, ny, nz, kk))
+        x = np.linalg.solve(M, b)
+        return 0.5 * np.sum((x - xo) ** 2), x, M
+
+    L0, x, M = loss(k)
+    lam = np.linalg.solve(M.T, x - xo)
+    g = O.diffusion7_kappa_grad(nx, ny, nz, k, x, lam)
+    h = 1e-6
+    fd = np.array([(loss(k + h * e)[0] - loss(k - h * e)[0]) / (2 * h) for e in np.eye(n)])
+    np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-9)
+
+
+def test_sddmm_brute_force():
+    rng = np.random.default_rng(5)
+    A, M = random_csr(20, 20, 0.3, rng)
+    lam, x = rng.normal(size=20), rng.normal(size=20)
+    g = O.sddmm_csr(A, lam, x)
+    full = -np.outer(lam, x)
+    rows = np.repeat(np.arange(20), np.diff(A.xadj))
+    np.testing.assert_array_equal(g, full[rows, A.cols - 1])
