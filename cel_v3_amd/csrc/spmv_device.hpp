@@ -39,7 +39,9 @@ __device__ __forceinline__ AT csr_tile(int n, int r0, const int* __restrict__ xa
 {
     const int tid = threadIdx.x;
     const int nrows = min(kTileRows, n - r0);
-    if (tid <= nrows) sm.ptr[tid] = xadj[r0 + tid];
+    // nrows + 1 row pointers: a full tile needs 257 of them, one more than
+    // the block has threads
+    for (int k = tid; k <= nrows; k += kBlock) sm.ptr[k] = xadj[r0 + k];
     __syncthreads();
     const int e0 = sm.ptr[0];
     const int cnt = sm.ptr[nrows] - e0;

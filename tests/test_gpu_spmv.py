@@ -58,9 +58,12 @@ def test_jad_conversion_bit_exact(gpu, name, A, mode):
         x = np.random.default_rng(3).normal(size=A.n)
         y = M.spmv(x, fmt=gpu.FMT_JAD)
         np.testing.assert_array_equal(y, O.spmv_jad(ref, x))
-        np.testing.assert_array_equal(y, O.spmv_csr(A, x))
-        xf = x.astype(np.float32)
-        np.testing.assert_array_equal(M.spmv_f32(xf, fmt=gpu.FMT_JAD), O.spmv_csr_f32(A, xf))
+        # the reference's own counts drop entries when the shortest row is
+        # non-empty (a 1-row matrix: ilo = 1, diagonal 1 gets #{len > 1} = 0)
+        if mode == O.JAD_SPARSKIT or ref["ilo"] == 0:
+            np.testing.assert_array_equal(y, O.spmv_csr(A, x))
+            xf = x.astype(np.float32)
+            np.testing.assert_array_equal(M.spmv_f32(xf, fmt=gpu.FMT_JAD), O.spmv_csr_f32(A, xf))
 
 
 def test_jad_undefined_reference_counts_refuse_spmv(gpu):
