@@ -1,0 +1,286 @@
+#!/usr/bin/env python3
+"""256^3 Jacobi-PCG time-to-1e-10 on one MI355X: mixed precision vs plain FP64,
+and SpMV bandwidth against the HBM roofline.
+
+    python bench.py [--steps K] [--warmup W] [--size N] [--format csr|jad]
+
+One step = one full solve of the 7-point Poisson system on an N^3 grid
+(diag 6, off-diagonal -1, Dirichlet outside), b = A x_true with
+x_true ~ U[0,1) (numpy default_rng(1)), x0 = 0, stopped at
+||b - A x|| <= tol * ||b||.  Inputs are resident in HBM before the timed
+region starts.  Prints one JSON line on stdout; progress goes to stderr.
+
+The FP64 solves run with the one-subdomain model of the reference's
+distributed SpMV: the 256^3 block carries six halo faces (65,536 entries
+each) and packs its six send faces every SpMV; with no neighbour present the
+received values are zero, so the solve is the Dirichlet problem.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "256^3 PCG time-to-1e-10: mixed vs plain FP64 GPU; SpMV GB/s as % HBM peak"
+HBM_PEAK_GBS = 8000.0
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_pcg_spmv_fp64_256.traffic")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def face_blocks(N):
+    """Halo blocks of the centre rank of a 3x3x3 decomposition with N^3 cells
+    per rank, in ascending neighbour rank order (z-, y-, x-, x+, y+, z+).
+    Returns [(proc, rows, cols)] with 1-based local ids."""
+    idx = np.arange(N * N, dtype=np.int64)
+    a, b = idx % N, idx // N
+    n2 = N * N
+
+    def cell(x, y, z):
+        return x + N * (y + N * z) + 1
+
+    faces = [
+        (4, cell(a, b, 0), cell(a, b, N - 1)),        # z-: neighbour's top plane
+        (10, cell(a, 0, b), cell(a, N - 1, b)),       # y-
+        (12, cell(0, a, b), cell(N - 1, a, b)),       # x-
+        (14, cell(N - 1, a, b), cell(0, a, b)),       # x+
+        (16, cell(a, N - 1, b), cell(a, 0, b)),       # y+
+        (22, cell(a, b, N - 1), cell(a, b, 0)),       # z+
+    ]
+    out = []
+    for proc, rows, cols in faces:
+        order = np.argsort(rows, kind="stable")
+        out.append((proc, rows[order].astype(np.int32), cols[order].astype(np.int32)))
+    del n2
+    return out
+
+
+def read_traffic():
+    """HBM bytes per K1 launch from the committed rocprofv3 PMC summary."""
+    try:
+        with open(PROFILE_TRAFFIC) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(N, iters_needed, budget_s=15.0):
+    """The oracle's OpenMP restatement of the reference loop (thread 0
+    idle, 4-aligned worker chunks) on the host cores, on a bounded sample
+    of iterations of the same 256^3 solve, extrapolated to the iteration
+    count of the FP64 GPU solve."""
+    import oracle as O
+
+    threads = len(os.sched_getaffinity(0))
+    threads = max(2, min(threads, int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
+    workers = threads - 1
+    t0 = time.time()
+    A = O.gen_poisson7(N, N, N)
+    xt = np.random.default_rng(1).random(A.n)
+    b = O.spmv_csr(A, xt, workers=workers)
+    jac = O.set_jacobi(O.csr_diagonal(A))
+    log(f"cpu baseline: matrix ready in {time.time() - t0:.1f}s, {threads} threads")
+    probe = 3
+    t0 = time.time()
+    O.pcg_jacobi(A, jac, b, x0_mode=O.X0_ZERO, eps=0.0, iter_max=probe, workers=workers)
+    t_probe = time.time() - t0
+    k = int(max(5, min(400, budget_s / max(t_probe / probe, 1e-6))))
+    t0 = time.time()
+    r = O.pcg_jacobi(A, jac, b, x0_mode=O.X0_ZERO, eps=0.0, iter_max=k, workers=workers)
+    dt = time.time() - t0
+    per_iter = dt / max(r["iters"], 1)
+    log(f"cpu baseline: {r['iters']} iterations in {dt:.1f}s ({per_iter * 1e3:.1f} ms/iter)")
+    return {
+        "value": per_iter * iters_needed,
+        "unit": "s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{r['iters']} Jacobi-PCG iterations of the {N}^3 FP64 solve "
+                   f"(oracle/cel_oracle.c, reference op order, {workers} workers + idle master "
+                   f"thread) in {dt:.2f} s, extrapolated to the {iters_needed} iterations "
+                   f"the FP64 GPU solve needs"),
+        "per_iter_ms": per_iter * 1e3,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--format", default="csr", choices=["csr", "jad"])
+    ap.add_argument("--skip-cpu", action="store_true", help="do not time the CPU baseline")
+    ap.add_argument("--only", default="all", choices=["all", "fp64", "mixed", "spmv"],
+                    help="run one part only (profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(the path's multi-GPU sharding is a design, not part of this build)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import cel_v3_amd as cel
+
+    N = args.size
+    cel.init(0)
+    dev, cus = cel.device_info()
+    log(f"device {dev}, {cus} CUs; building {N}^3 Poisson on the device")
+    M = cel.Matrix.poisson7(N, N, N)
+    M.set_jacobi()
+    fmt = cel.FMT_CSR
+    if args.format == "jad":
+        M.to_jad()
+        fmt = cel.FMT_JAD
+    use_halo = int(args.format == "csr")
+    if use_halo:
+        for proc, rows, cols in face_blocks(N):
+            M.add_halo_block(proc, proc * N ** 3 + 1, N ** 3, rows, cols, -np.ones(rows.size))
+            M.set_send_list(proc, rows)
+    xt = np.random.default_rng(1).random(M.n)
+    d_xt = cel.DeviceArray.from_host(xt)
+    d_b = cel.DeviceArray(M.n)
+    M.spmv(d_xt, d_b)
+    bnorm = float(np.linalg.norm(d_b.to_host()))
+    log(f"n={M.n} nnz={M.nnz} |b|={bnorm:.6e}")
+
+    def solve(precision, profile=0, halo=0):
+        x = cel.DeviceArray(M.n)
+        r = M.pcg(d_b, x0=None, tol=args.tol, tol_mode=cel.TOL_REL_B, x0_mode=cel.X0_ZERO,
+                  iter_max=20000, precision=precision, fmt=fmt, profile=profile, use_halo=halo,
+                  allow_not_converged=False)
+        return r
+
+    def check(r):
+        x = r["x"]
+        res = cel.DeviceArray(M.n)
+        M.spmv(x, res)
+        xh = x.to_host()
+        rel = float(np.linalg.norm(d_b.to_host() - res.to_host()) / bnorm)
+        err = float(np.abs(xh - xt).max())
+        return rel, err
+
+    out = {}
+    # ------------------------------------------------------------ SpMV kernels alone
+    if args.only in ("all", "spmv"):
+        log("SpMV kernels")
+        x64 = cel.DeviceArray.from_host(xt)
+        y64 = cel.DeviceArray(M.n)
+        x32 = cel.DeviceArray.from_host(xt.astype(np.float32))
+        y32 = cel.DeviceArray(M.n, np.float32)
+        spmv = {}
+        for name, f, prec, x, y in (("csr_fp64", cel.FMT_CSR, cel.PREC_FP64, x64, y64),
+                                    ("csr_fp32", cel.FMT_CSR, cel.PREC_MIXED, x32, y32)):
+            ms, nb = M.spmv_time(x, y, fmt=f, precision=prec, reps=20)
+            spmv[name] = {"ms": ms, "bytes": nb, "GBs": nb / ms * 1e-6,
+                          "frac_hbm": nb / ms * 1e-6 / HBM_PEAK_GBS}
+        if fmt == cel.FMT_JAD:
+            ms, nb = M.spmv_time(x64, y64, fmt=cel.FMT_JAD, precision=cel.PREC_FP64, reps=20)
+            spmv["jad_fp64"] = {"ms": ms, "bytes": nb, "GBs": nb / ms * 1e-6,
+                                "frac_hbm": nb / ms * 1e-6 / HBM_PEAK_GBS}
+        cb = 2 * 1024 ** 3
+        ms = cel.stream_copy_time(cb, reps=10)
+        spmv["stream_copy_GBs"] = 2 * cb / ms * 1e-6
+        for k, v in spmv.items():
+            log(f"  {k}: {v if not isinstance(v, dict) else round(v['GBs'], 1)} GB/s")
+        out["spmv"] = spmv
+        del x64, y64, x32, y32
+
+    if args.only == "spmv":
+        print(json.dumps(out))
+        return
+
+    # ------------------------------------------------------------ warm-up
+    for w in range(args.warmup):
+        log(f"warm-up {w + 1}/{args.warmup}")
+        if args.only in ("all", "fp64"):
+            solve(cel.PREC_FP64, halo=use_halo)
+        if args.only in ("all", "mixed"):
+            solve(cel.PREC_MIXED)
+
+    # ------------------------------------------------------------ timed region
+    fp64_runs, mixed_runs = [], []
+    cel.lib().cel_gpu_synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        if args.only in ("all", "mixed"):
+            r = solve(cel.PREC_MIXED)
+            mixed_runs.append(r)
+            log(f"step {k + 1}: mixed {r['solve_ms']:.1f} ms, {r['iters']} inner iterations, "
+                f"{r['outer_iters']} FP64 replacements")
+    cel.lib().cel_gpu_synchronize()
+    t_mixed = time.perf_counter() - t_start
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        if args.only in ("all", "fp64"):
+            r = solve(cel.PREC_FP64, profile=1, halo=use_halo)
+            fp64_runs.append(r)
+            log(f"step {k + 1}: fp64 {r['solve_ms']:.1f} ms, {r['iters']} iterations")
+    cel.lib().cel_gpu_synchronize()
+    t_fp64 = time.perf_counter() - t_start
+
+    res = dict(metric=METRIC, n_gpus=1, steps=args.steps, warmup=args.warmup,
+               higher_is_better=False, scaling="weak", vs_baseline=None, data="synthetic",
+               unit="s")
+    cfg = {"workload": f"poisson7_{N}^3_jacobi_pcg_tol{args.tol:g}", "n": M.n, "nnz": M.nnz,
+           "format": args.format, "x_true": "U[0,1) numpy default_rng(1)", "x0": 0,
+           "stop": "||b-Ax|| <= tol*||b||", "halo_faces_fp64": 6 if use_halo else 0,
+           "parallelism": "1 GPU, one subdomain"}
+    if mixed_runs:
+        ms = [r["solve_ms"] for r in mixed_runs]
+        rel, err = check(mixed_runs[-1])
+        res["value"] = float(np.median(ms)) / 1e3
+        res["ms_per_step"] = t_mixed / args.steps * 1e3
+        res["dtype"] = "f32+f64"
+        res["mixed"] = {"solve_ms": ms, "inner_iters": mixed_runs[-1]["iters"],
+                        "fp64_replacements": mixed_runs[-1]["outer_iters"],
+                        "true_rel_residual": rel, "max_abs_err_vs_x_true": err,
+                        "wall_ms_per_step": t_mixed / args.steps * 1e3}
+        log(f"mixed: {res['value']:.4f} s, true rel residual {rel:.3e}, max err {err:.3e}")
+    if fp64_runs:
+        ms = [r["solve_ms"] for r in fp64_runs]
+        rel, err = check(fp64_runs[-1])
+        last = fp64_runs[-1]
+        k1_ms = last["spmv_ms"] / max(last["spmv_launches"], 1)
+        achieved = last["spmv_bytes"] / (k1_ms * 1e-3) / 1e9
+        traffic = read_traffic()
+        res["fp64"] = {"solve_ms": ms, "iters": last["iters"], "true_rel_residual": rel,
+                       "max_abs_err_vs_x_true": err, "wall_ms_per_step": t_fp64 / args.steps * 1e3,
+                       "iter_bytes": last["iter_bytes"],
+                       "iter_GBs": last["iter_bytes"] * last["iters"] / (ms[-1] * 1e-3) / 1e9}
+        res["roofline"] = {"kernel": "k_pcg_spmv<double,CSR,halo> (v = A p fused with v.p)",
+                           "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                           "algorithmic_bytes": last["spmv_bytes"], "mean_launch_ms": k1_ms,
+                           "launches": last["spmv_launches"]}
+        log(f"fp64: {np.median(ms) / 1e3:.4f} s, {last['iters']} iterations, K1 {k1_ms * 1e3:.1f} us "
+            f"= {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%} of 8 TB/s)")
+        if "value" not in res:
+            res["value"] = float(np.median(ms)) / 1e3
+            res["ms_per_step"] = t_fp64 / args.steps * 1e3
+            res["dtype"] = "f64"
+    if mixed_runs and fp64_runs:
+        res["speedup_mixed_vs_fp64"] = (np.median([r["solve_ms"] for r in fp64_runs]) /
+                                        np.median([r["solve_ms"] for r in mixed_runs]))
+    res["config"] = cfg
+    res.update(out)
+    if not args.skip_cpu and fp64_runs and args.only == "all":
+        res["cpu_baseline"] = cpu_baseline(N, fp64_runs[-1]["iters"])
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

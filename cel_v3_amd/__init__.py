@@ -38,6 +38,7 @@ EXPORTS = [
     "cel_gpu_mat_get_block", "cel_gpu_mat_set_send_list", "cel_gpu_face_pack",
     "cel_gpu_spmv_distr", "cel_gpu_mat_ghost_len", "cel_gpu_pcg_params_default",
     "cel_gpu_pcg_solve", "cel_gpu_cg_adjoint", "cel_gpu_diffusion7_kappa_grad",
+    "cel_gpu_mat_add_halo_block", "cel_gpu_stream_copy_time",
 ]
 
 
@@ -52,7 +53,7 @@ class PcgParams(C.Structure):
                 ("x0_mode", C.c_int), ("precision", C.c_int), ("format", C.c_int),
                 ("memspace", C.c_int), ("use_halo", C.c_int), ("ru_delta", C.c_double),
                 ("ru_restart", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int),
-                ("profile", C.c_int)]
+                ("profile", C.c_int), ("alpha_stop", C.c_int)]
 
 
 class PcgStats(C.Structure):
@@ -309,6 +310,12 @@ class Matrix:
         return dict(proc=proc.value, rows=rows[:k], cols=cols[:k], vals=vals[:k], xadj=xadj,
                     recv_perm=perm[: rlen.value])
 
+    def add_halo_block(self, proc, proc_first_row, proc_len, rows, cols, vals, index_base=1):
+        rows, cols, vals = _i32(rows), _i32(cols), _f64(vals)
+        _check(lib().cel_gpu_mat_add_halo_block(self.h, proc, proc_first_row, proc_len, rows.size,
+                                                _ptr(rows), _ptr(cols), _ptr(vals), index_base))
+        self.nblocks += 1
+
     def set_send_list(self, peer, perm):
         perm = _i32(perm)
         _check(lib().cel_gpu_mat_set_send_list(self.h, peer, perm.size, _ptr(perm)))
@@ -358,8 +365,10 @@ class Matrix:
             out["hist"] = hist[: st.iters + 1]
         return out
 
-    def adjoint(self, x, dLdx, want_dLda=True, tol=1e-12, tol_mode=TOL_REL_B, iter_max=10000, **kw):
-        p = params(tol=tol, tol_mode=tol_mode, iter_max=iter_max, x0_mode=X0_ZERO, **kw)
+    def adjoint(self, x, dLdx, want_dLda=True, tol=1e-12, tol_mode=TOL_REL_B, iter_max=10000,
+                alpha_stop=0, **kw):
+        p = params(tol=tol, tol_mode=tol_mode, iter_max=iter_max, x0_mode=X0_ZERO,
+                   alpha_stop=alpha_stop, **kw)
         p.memspace = MEM_HOST
         st = PcgStats()
         lam = np.zeros(self.n)
@@ -378,6 +387,12 @@ def params(**kw) -> PcgParams:
             raise TypeError(f"unknown PCG parameter {k}")
         setattr(p, k, v)
     return p
+
+
+def stream_copy_time(nbytes, reps=20):
+    ms = C.c_double()
+    _check(lib().cel_gpu_stream_copy_time(C.c_size_t(nbytes), reps, C.byref(ms)))
+    return ms.value
 
 
 def kappa_grad(nx, ny, nz, kappa, x, lam):

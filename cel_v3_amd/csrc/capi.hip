@@ -561,6 +561,75 @@ extern "C" int cel_gpu_spmv_time(cel_gpu_mat* A, int format, int precision, cons
 }
 
 // ------------------------------------------------------------------ distributed split
+// Receive lists, ghost slots and the merged per-row halo arrays of every
+// halo block of M (cel_sp_mat_distr_module.f90:232-280 for the lists).
+// Block b's receive list is the ascending list of its referenced columns in
+// the owner's local numbering; ghost slots are positions in the
+// concatenation of all receive lists in block order.
+static int rebuild_halo(Mat& M)
+{
+    const int n = M.n;
+    const int my_off = M.row_offset;
+    const int nb = 1 + (int)M.halo.size();
+    int ghost = 0;
+    std::vector<std::vector<int>> slot(nb);
+    for (int b = 1; b < nb; ++b) {
+        HaloBlock& H = M.halo[b - 1];
+        const int plen = H.proc_len;
+        std::vector<int> pos((size_t)std::max(plen, 1), -1);
+        for (int c : H.h_cols) pos[c - H.col_offset - 1] = 0;
+        H.h_recv_perm.clear();
+        for (int j = 0; j < plen; ++j)
+            if (pos[j] == 0) {
+                pos[j] = (int)H.h_recv_perm.size();
+                H.h_recv_perm.push_back(j + 1);
+            }
+        H.recv_len = (int)H.h_recv_perm.size();
+        H.recv_base = ghost;
+        ghost += H.recv_len;
+        H.h_xadj.assign((size_t)n + 1, 0);
+        H.h_xadj[0] = 1;
+        std::vector<int> cnt((size_t)n, 0);
+        for (int r : H.h_rows) cnt[r - my_off - 1] += 1;
+        for (int i = 1; i <= n; ++i) H.h_xadj[i] = H.h_xadj[i - 1] + cnt[i - 1];
+        slot[b].resize(H.h_cols.size());
+        for (size_t k = 0; k < H.h_cols.size(); ++k)
+            slot[b][k] = H.recv_base + pos[H.h_cols[k] - H.col_offset - 1];
+    }
+    M.ghost_len = ghost;
+    // merged halo entries per local row: block order, then entry order
+    std::vector<int> rowcnt((size_t)n, 0);
+    for (int b = 1; b < nb; ++b)
+        for (int r : M.halo[b - 1].h_rows) rowcnt[r - my_off - 1] += 1;
+    std::vector<int> hrows, hptr(1, 0);
+    std::vector<int> rowq((size_t)n, -1);
+    for (int i = 0; i < n; ++i)
+        if (rowcnt[i]) {
+            rowq[i] = (int)hrows.size();
+            hrows.push_back(i);
+            hptr.push_back(hptr.back() + rowcnt[i]);
+        }
+    std::vector<int> fillp(hptr.begin(), hptr.end() - 1);
+    std::vector<int> hcols((size_t)hptr.back());
+    std::vector<double> hvals((size_t)hptr.back());
+    for (int b = 1; b < nb; ++b) {
+        const HaloBlock& H = M.halo[b - 1];
+        for (size_t k = 0; k < H.h_rows.size(); ++k) {
+            const int q = rowq[H.h_rows[k] - my_off - 1];
+            const int at = fillp[q]++;
+            hcols[at] = slot[b][k];
+            hvals[at] = H.h_vals[k];
+        }
+    }
+    M.halo_nrows = (int)hrows.size();
+    CEL_TRY(upload(M.halo_rows, hrows.data(), hrows.size()));
+    CEL_TRY(upload(M.halo_ptr, hptr.data(), hptr.size()));
+    CEL_TRY(upload(M.halo_cols, hcols.data(), hcols.size()));
+    CEL_TRY(upload(M.halo_vals, hvals.data(), hvals.size()));
+    CEL_HIP(hipStreamSynchronize(rt().stream));
+    return CEL_OK;
+}
+
 extern "C" int cel_gpu_mat_distr_split(cel_gpu_mat** A, int nranks, const int* vtxdist, int my_rank,
                                        int nnz, const int* rows, const int* cols,
                                        const double* vals, int index_base)
@@ -617,6 +686,7 @@ extern "C" int cel_gpu_mat_distr_split(cel_gpu_mat** A, int nranks, const int* v
         HaloBlock& H = M.halo[b - 1];
         H.proc = procs[b];
         H.col_offset = vd[procs[b]] - 1;
+        H.proc_len = vd[procs[b] + 1] - vd[procs[b]];
         H.nnz = per_rank[procs[b]];
         H.h_rows.reserve(H.nnz);
         H.h_cols.reserve(H.nnz);
@@ -653,65 +723,75 @@ extern "C" int cel_gpu_mat_distr_split(cel_gpu_mat** A, int nranks, const int* v
     // the reference keeps the diagonal captured during the split
     rc = upload(M.diag, M.h_diag_split.data(), (size_t)n);
     if (rc < 0) { delete h; return rc; }
-    // halo blocks: receive lists (ascending referenced columns) and ghost slots
-    int ghost = 0;
-    std::vector<std::vector<int>> slot(nb);
-    for (int b = 1; b < nb; ++b) {
-        HaloBlock& H = M.halo[b - 1];
-        const int plen = vd[H.proc + 1] - vd[H.proc];
-        std::vector<int> pos((size_t)plen, -1);
-        for (int c : H.h_cols) pos[c - H.col_offset - 1] = 0;
-        H.h_recv_perm.clear();
-        for (int j = 0; j < plen; ++j)
-            if (pos[j] == 0) {
-                pos[j] = (int)H.h_recv_perm.size();
-                H.h_recv_perm.push_back(j + 1);
-            }
-        H.recv_len = (int)H.h_recv_perm.size();
-        H.recv_base = ghost;
-        ghost += H.recv_len;
-        H.h_xadj.assign((size_t)n + 1, 0);
-        H.h_xadj[0] = 1;
-        std::vector<int> cnt((size_t)n, 0);
-        for (int r : H.h_rows) cnt[r - my_off - 1] += 1;
-        for (int i = 1; i <= n; ++i) H.h_xadj[i] = H.h_xadj[i - 1] + cnt[i - 1];
-        slot[b].resize(H.h_cols.size());
-        for (size_t k = 0; k < H.h_cols.size(); ++k)
-            slot[b][k] = H.recv_base + pos[H.h_cols[k] - H.col_offset - 1];
-    }
-    M.ghost_len = ghost;
-    // merged halo entries per local row: block order, then entry order
-    std::vector<int> rowcnt((size_t)n, 0);
-    for (int b = 1; b < nb; ++b)
-        for (int r : M.halo[b - 1].h_rows) rowcnt[r - my_off - 1] += 1;
-    std::vector<int> hrows, hptr(1, 0);
-    std::vector<int> rowq((size_t)n, -1);
-    for (int i = 0; i < n; ++i)
-        if (rowcnt[i]) {
-            rowq[i] = (int)hrows.size();
-            hrows.push_back(i);
-            hptr.push_back(hptr.back() + rowcnt[i]);
-        }
-    std::vector<int> fillp(hptr.begin(), hptr.end() - 1);
-    std::vector<int> hcols((size_t)hptr.back());
-    std::vector<double> hvals((size_t)hptr.back());
-    for (int b = 1; b < nb; ++b) {
-        const HaloBlock& H = M.halo[b - 1];
-        for (size_t k = 0; k < H.h_rows.size(); ++k) {
-            const int q = rowq[H.h_rows[k] - my_off - 1];
-            const int at = fillp[q]++;
-            hcols[at] = slot[b][k];
-            hvals[at] = H.h_vals[k];
-        }
-    }
-    M.halo_nrows = (int)hrows.size();
-    rc = upload(M.halo_rows, hrows.data(), hrows.size());
-    if (rc == CEL_OK) rc = upload(M.halo_ptr, hptr.data(), hptr.size());
-    if (rc == CEL_OK) rc = upload(M.halo_cols, hcols.data(), hcols.size());
-    if (rc == CEL_OK) rc = upload(M.halo_vals, hvals.data(), hvals.size());
-    if (rc == CEL_OK && hipStreamSynchronize(rt().stream) != hipSuccess) rc = fail(CEL_ERR_HIP, "upload failed");
+    rc = rebuild_halo(M);
     if (rc < 0) { delete h; return rc; }
     *A = h;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_mat_add_halo_block(cel_gpu_mat* A, int proc, int proc_first_row,
+                                          int proc_len, int nnz, const int* rows, const int* cols,
+                                          const double* vals, int index_base)
+{
+    if (!A || proc < 0 || proc_first_row < 1 || proc_len < 1 || nnz < 0 ||
+        (nnz > 0 && (!rows || !cols || !vals)) || (index_base != 0 && index_base != 1))
+        return fail(CEL_ERR_ARG, "bad arguments to cel_gpu_mat_add_halo_block");
+    CEL_TRY(ensure_init());
+    Mat& M = A->m;
+    if (!M.halo.empty() && M.halo.back().proc >= proc)
+        return fail(CEL_ERR_ARG, "halo blocks must be added in ascending rank order");
+    HaloBlock H;
+    H.proc = proc;
+    H.col_offset = proc_first_row - 1;
+    H.proc_len = proc_len;
+    H.nnz = nnz;
+    int prev = -1;
+    for (int k = 0; k < nnz; ++k) {
+        const int lr = rows[k] - index_base, lc = cols[k] - index_base;
+        if (lr < 0 || lr >= M.n || lc < 0 || lc >= proc_len)
+            return fail(CEL_ERR_BAD_MATRIX, "halo entry out of range at " + std::to_string(k));
+        if (lr < prev) return fail(CEL_ERR_BAD_MATRIX, "halo entries are not sorted by row");
+        prev = lr;
+        H.h_rows.push_back(lr + M.row_offset + 1);
+        H.h_cols.push_back(lc + H.col_offset + 1);
+        H.h_vals.push_back(vals[k]);
+    }
+    M.halo.push_back(std::move(H));
+    return rebuild_halo(M);
+}
+
+__global__ void k_stream_copy(size_t n4, const double2* __restrict__ src, double2* __restrict__ dst)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" int cel_gpu_stream_copy_time(size_t bytes, int reps, double* mean_ms)
+{
+    if (reps < 1 || bytes < 16) return fail(CEL_ERR_ARG, "bad arguments to cel_gpu_stream_copy_time");
+    CEL_TRY(ensure_init());
+    const size_t n = bytes / 16;
+    DevBuf<double2> a, b;
+    CEL_TRY(a.alloc(n));
+    CEL_TRY(b.alloc(n));
+    hipStream_t s = rt().stream;
+    CEL_HIP(hipMemsetAsync(a.p, 0, n * 16, s));
+    const int G = rt().num_cus * 8;
+    hipLaunchKernelGGL(k_stream_copy, dim3(G), dim3(256), 0, s, n, a.p, b.p);
+    hipEvent_t e0, e1;
+    CEL_HIP(hipEventCreate(&e0));
+    CEL_HIP(hipEventCreate(&e1));
+    CEL_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i)
+        hipLaunchKernelGGL(k_stream_copy, dim3(G), dim3(256), 0, s, n, a.p, b.p);
+    CEL_HIP(hipEventRecord(e1, s));
+    CEL_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    CEL_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (mean_ms) *mean_ms = ms / reps;
     return CEL_OK;
 }
 
@@ -834,6 +914,7 @@ extern "C" void cel_gpu_pcg_params_default(cel_pcg_params* p)
     p->check_every = 16;
     p->use_graph = 0;
     p->profile = 0;
+    p->alpha_stop = 1;       // cel_cgalg_module.f90:248
 }
 
 extern "C" int cel_gpu_pcg_solve(cel_gpu_mat* A, const double* b, double* x,
@@ -862,6 +943,7 @@ extern "C" int cel_gpu_cg_adjoint(cel_gpu_mat* A, const double* x, const double*
     else {
         cel_gpu_pcg_params_default(&p);
         p.x0_mode = CEL_X0_ZERO;
+        p.alpha_stop = 0;
     }
     // A is symmetric for every matrix this path generates; A^T lambda = dLdx
     // is solved with the same PCG

@@ -84,6 +84,7 @@ struct HaloBlock {
     int proc = -1;        // owning rank of the columns
     int nnz = 0;
     int col_offset = 0;   // vtxdist(proc+1)-1, as the reference stores it
+    int proc_len = 0;     // rows owned by proc
     int recv_len = 0;     // number of distinct referenced columns
     int recv_base = 0;    // offset of this block's slots in the ghost region
     // host copies kept for getters (reference layout, 1-based, global ids)

@@ -8,25 +8,32 @@
 //     x = x + sigma p ; r = r + (-sigma) v ; z = jac*r
 //     beta = (r.z)/alpha ; alpha = r.z ; p = beta p + z ; norm2 = ||r||
 //
-// One iteration is three kernels, each ending in a deterministic grid
-// reduction whose last block updates the scalars on the device, so the host
-// never waits inside the loop:
-//   K1  v = A p (CSR tile from LDS or JAD), partial v.p   -> sigma
-//   K2  r = r + (-sigma) v, z = jac*r, partials r.z, r.r -> beta, alpha,
-//       norm2, k, stop decision
-//   K3  x = x + sigma p ; p = beta p + jac*r   (x update moved here: it
-//       reads p anyway, which saves one pass over p; x never feeds back
-//       into the iteration, and each element is computed by the same
-//       formula as the reference)
-// Per element the arithmetic is the reference's (products rounded before
-// the add, compiled with -ffp-contract=off); only the reduction order of
-// the dot products differs.
+// One iteration is three streaming kernels:
+//   K1  v = A p (LDS-staged CSR tile or JAD), one partial of v.p per block
+//   K2  sigma = alpha / sum(v.p) ; r = r + (-sigma) v ; z = jac*r ;
+//       partials of r.z and r.r
+//   K3  beta = sum(r.z)/alpha ; x = x + sigma p ; p = beta p + jac*r ;
+//       alpha, norm2, k, loop decision
+// (the x update sits in K3 because K3 reads p anyway; x never feeds back
+// into the iteration and each element is the reference's formula).
 //
-// Kernels of iteration k check device flags instead of k, so one captured
-// chunk of iterations can be replayed as a hipGraph and runs past
-// convergence as no-ops.
+// No kernel waits on another block of the same launch: there are no grid
+// barriers, atomics or device-scope fences in the loop.  The scalars travel
+// as a small token along the ring of kernels -- every kernel reads the token
+// its predecessor wrote and block 0 writes the token for its successor -- and
+// a kernel that needs a dot product sums the predecessor's per-block
+// partials itself, in every block, in the same fixed order.  All blocks
+// therefore hold bit-identical scalars, the sums do not depend on which
+// block finished first, and the host never waits inside the loop.  A
+// stopped token is forwarded unchanged, so a captured chunk of iterations
+// can be replayed as a hipGraph and runs past convergence as no-ops.
+//
+// Per element the arithmetic is the reference's (products rounded before the
+// add, -ffp-contract=off); only the summation order of the dot products
+// differs from the reference (fixed tree here, worker chunks there).
 #include <algorithm>
 #include <cmath>
+#include <initializer_list>
 #include <vector>
 
 #include "cel_internal.hpp"
@@ -34,36 +41,71 @@
 
 namespace cel {
 
-struct PcgState {
-    double alpha, sigma, beta, norm2, eps, norm0, bnorm, tol, small, rnorm_ref, ru_delta;
-    double vp;
-    int iter, iter_max, stop, status, pending_xp, ru_flag, outer, tol_mode, ru_restart, hist_cap;
-    double* hist;
-    unsigned ctr[4];
+// Token passed along the kernel ring.
+struct Ctrl {
+    double alpha, sigma, beta, norm2, eps, norm0, bnorm, rnorm_ref;
+    int run;     // 1 while the loop continues
+    int iter;    // completed iterations
+    int status;  // 0 or a CEL_ERR_* code
+    int reduce;  // partials the next K1 must fold in first: 0 none, 1 set-up, 2 replacement
+    int ru;      // mixed: replace the residual at the end of this iteration
+    int outer;   // mixed: replacements done
+    int pad[2];
 };
 
-// ------------------------------------------------------------------ helpers
-__device__ __forceinline__ bool loop_continues(const PcgState* s)
+// Constant solver parameters.
+struct Prm {
+    double tol, small, ru_delta;
+    int tol_mode, iter_max, ru_restart, hist_cap, alpha_stop;
+    double* hist;
+};
+
+// the reference's absolute test |alpha| > 1.1 eps_mach (optional)
+__device__ __forceinline__ bool alpha_ok(const Prm& P, double alpha)
 {
-    return s->iter < s->iter_max && fabs(s->alpha) > s->small && s->norm2 > s->eps;
+    return !P.alpha_stop || fabs(alpha) > P.small;
 }
 
-__device__ __forceinline__ double eps_from(const PcgState* s, double r0)
+enum { kReduceNone = 0, kReduceSetup = 1, kReduceReplace = 2 };
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ double eps_from(const Prm& P, double bnorm, double r0)
 {
-    if (s->tol_mode == CEL_TOL_REL_B) return s->tol * s->bnorm;
-    if (s->tol_mode == CEL_TOL_REL_R0) return s->tol * r0;
-    return s->tol;
+    if (P.tol_mode == CEL_TOL_REL_B) return P.tol * bnorm;
+    if (P.tol_mode == CEL_TOL_REL_R0) return P.tol * r0;
+    return P.tol;
+}
+
+// Every thread of the block reads the token (written by the previous kernel).
+__device__ __forceinline__ Ctrl read_ctrl(const Ctrl* c) { return *c; }
+
+__device__ __forceinline__ void write_ctrl(Ctrl* dst, const Ctrl& c)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dst = c;
+}
+
+// Sum of G partials in fixed order, returned to every thread of the block.
+__device__ __forceinline__ double all_sum(const double* __restrict__ part, int G, double* red,
+                                          double* bcast)
+{
+    const double s = sum_partials(part, G, red);
+    if (threadIdx.x == 0) *bcast = s;
+    __syncthreads();
+    const double v = *bcast;
+    __syncthreads();
+    return v;
+}
+
+// Balanced contiguous ranges: logical block lb of G owns units [u0, u1).
+__device__ __forceinline__ void unit_range(long long nunits, int G, int lb, long long& u0,
+                                           long long& u1)
+{
+    u0 = nunits * lb / G;
+    u1 = nunits * (lb + 1) / G;
 }
 
 // ------------------------------------------------------------------ set-up
 __global__ void k_fill(long long n, double* __restrict__ x, double v)
-{
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x)
-        x[i] = v;
-}
-
-__global__ void k_fill_f32(long long n, float* __restrict__ x, float v)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x)
@@ -97,7 +139,7 @@ __device__ __forceinline__ const double* jad_vals<double>(const MatView& A) { re
 template <>
 __device__ __forceinline__ const float* jad_vals<float>(const MatView& A) { return A.jvals32; }
 
-// Row sums of one 256-row tile in the requested format.  Returns the sum of
+// Row sum of one 256-row tile in the requested format.  Returns the sum of
 // row `row` (set here).  For CSR row = r0 + tid; for JAD row = perm[r0+tid].
 // HALO adds the halo entries after the local sum ((0 + local) + halo...).
 template <typename VT, typename AT, int FMT, bool HALO>
@@ -124,309 +166,358 @@ __device__ __forceinline__ AT tile_rows(const MatView& A, int t, const VT* __res
     return s;
 }
 
-// Every reduction kernel below walks the chunks it owns; a chunk is
-// chunk_tiles tiles of 256 rows (tile kernels) or the same rows as plain
-// elements (vector kernels), and writes one partial per chunk.
-
-// r = b - A x (FP64 operator), p = jac*r, partials: r.p (or r32.p32), r.r, b.b.
-// OUT_T = double (FP64 form: r, p stored in FP64) or float (mixed form:
-// r rounded to FP32, p = jac32 * r32, the FP64 norm taken before rounding).
-template <typename OUT_T, int FMT, bool HALO>
+// ------------------------------------------------------------------ residual kernels
+// r = b - A x with the FP64 operator; per-block partials: r.z, r.r, b.b.
+// FP64 form: r and p = z = jac*r stored in FP64.  Mixed form (OUT_T =
+// float): r rounded to FP32, z = jac32 * r32 (p = z at the start or on a
+// restart), e = 0 at the start, the norm taken in FP64 before rounding.
+// The next K1 folds the partials into the token.
+template <typename OUT_T, int FMT, bool HALO, bool REPLACE>
 __global__ __launch_bounds__(kBlock) void k_residual(MatView A, const double* __restrict__ x,
                                                      const double* __restrict__ b,
                                                      const double* __restrict__ jac,
                                                      const float* __restrict__ jac32,
                                                      OUT_T* __restrict__ r, OUT_T* __restrict__ p,
-                                                     float* __restrict__ e, double* __restrict__ part,
-                                                     PcgState* __restrict__ st, int nchunks,
-                                                     int chunk_tiles)
+                                                     float* __restrict__ e,
+                                                     double* __restrict__ part,
+                                                     const Ctrl* __restrict__ cin,
+                                                     Ctrl* __restrict__ cout, Prm P)
 {
+    Ctrl c = read_ctrl(cin);
+    if (REPLACE) {
+        if (!c.run || !c.ru) {
+            write_ctrl(cout, c);
+            return;
+        }
+    }
     __shared__ CsrStage<double> sm;
     __shared__ int smap[kBlock];
     __shared__ double red[3 * kWaves];
-    __shared__ int bad;
-    if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
     const int lb = logical_block();
-    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
-    int c0, c1;
-    chunk_range(nchunks, gridDim.x, lb, c0, c1);
-    for (int c = c0; c < c1; ++c) {
-        double arp = 0.0, arr = 0.0, abb = 0.0;
-        const int t1 = min((c + 1) * chunk_tiles, ntiles);
-        for (int t = c * chunk_tiles; t < t1; ++t) {
-            int row;
-            double s = tile_rows<double, double, FMT, HALO>(A, t, x, sm, smap, row);
-            if (row < A.n) {
-                const double bi = b[row];
-                if (!isfinite(bi)) bad = 1;
-                const double ri = bi - s;  // r = 0; r = r + A x; r = b - r
-                double bb = bi * bi;
-                abb = abb + bb;
-                double rr = ri * ri;
-                arr = arr + rr;
-                if constexpr (sizeof(OUT_T) == 8) {
-                    const double pi = ri * jac[row];
-                    r[row] = ri;
-                    p[row] = pi;
-                    double rp = ri * pi;
-                    arp = arp + rp;
-                } else {
-                    const float rf = (float)ri;
-                    const float pf = rf * jac32[row];
-                    r[row] = rf;
-                    p[row] = pf;
-                    e[row] = 0.0f;
-                    double rp = (double)rf * (double)pf;
-                    arp = arp + rp;
-                }
+    const int G = gridDim.x;
+    const long long ntiles = (A.n + kTileRows - 1) / kTileRows;
+    long long t0, t1;
+    unit_range(ntiles, G, lb, t0, t1);
+    const bool restart = REPLACE && P.ru_restart;
+    double arz = 0.0, arr = 0.0, abb = 0.0;
+    for (long long t = t0; t < t1; ++t) {
+        int row;
+        double s = tile_rows<double, double, FMT, HALO>(A, (int)t, x, sm, smap, row);
+        if (row < A.n) {
+            const double bi = b[row];
+            const double ri = bi - s;  // r = 0; r = r + A x; r = b - r
+            double bb = bi * bi;  // a non-finite b makes this sum non-finite
+            abb = abb + bb;
+            double rr = ri * ri;
+            arr = arr + rr;
+            if constexpr (sizeof(OUT_T) == 8) {
+                const double zi = ri * jac[row];
+                r[row] = ri;
+                p[row] = zi;
+                double rz = ri * zi;
+                arz = arz + rz;
+            } else {
+                const float rf = (float)ri;
+                const float zf = rf * jac32[row];
+                r[row] = rf;
+                if (!REPLACE || restart) p[row] = zf;
+                if (!REPLACE) e[row] = 0.0f;
+                double rz = (double)rf * (double)zf;
+                arz = arz + rz;
             }
         }
-        block_sum2(arp, arr, red);
-        const double sbb = block_sum(abb, red + 2 * kWaves);
-        if (threadIdx.x == 0) {
-            part[c] = arp;
-            part[nchunks + c] = arr;
-            part[2 * nchunks + c] = sbb;
-        }
     }
-    __syncthreads();
-    if (threadIdx.x == 0 && bad) st->status = CEL_ERR_NONFINITE;
-    if (!last_block(&st->ctr[0])) return;
-    const double rp = sum_partials(part, nchunks, red);
-    const double rr = sum_partials(part + nchunks, nchunks, red);
-    const double bb = sum_partials(part + 2 * nchunks, nchunks, red);
+    block_sum2(arz, arr, red);
+    const double sbb = block_sum(abb, red + 2 * kWaves);
     if (threadIdx.x == 0) {
-        const double rn = sqrt(rr);
-        st->bnorm = sqrt(bb);
-        st->alpha = rp;
-        st->norm2 = rn;
-        st->norm0 = rn;
-        st->eps = eps_from(st, rn);
-        st->rnorm_ref = rn;
-        st->iter = 0;
-        st->outer = 0;
-        st->pending_xp = 0;
-        st->ru_flag = 0;
-        if (st->hist_cap > 0) st->hist[0] = rn;
-        st->stop = (st->status != 0 || !loop_continues(st)) ? 1 : 0;
+        part[lb] = arz;
+        part[G + lb] = arr;
+        part[2 * G + lb] = sbb;
     }
+    c.reduce = REPLACE ? kReduceReplace : kReduceSetup;
+    c.ru = 0;
+    write_ctrl(cout, c);
 }
 
-// K1: v = A p, partial v.p ; last block: sigma = alpha / (v.p)
+// K1: v = A p, per-block partial of v.p.  Folds pending set-up or
+// replacement partials into the token first (every block, same order) and
+// decides whether the iteration runs.
 template <typename VT, int FMT, bool HALO>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(MatView A, const VT* __restrict__ p,
                                                      VT* __restrict__ v, double* __restrict__ part,
-                                                     PcgState* __restrict__ st, int nchunks,
-                                                     int chunk_tiles)
+                                                     const double* __restrict__ rpart, int Gr,
+                                                     const Ctrl* __restrict__ cin,
+                                                     Ctrl* __restrict__ cout, Prm P)
 {
-    if (st->stop) return;
+    __shared__ double red[kWaves];
+    __shared__ double bcast;
+    Ctrl c = read_ctrl(cin);
+    if (c.reduce != kReduceNone) {
+        const double rz = all_sum(rpart, Gr, red, &bcast);
+        const double rr = all_sum(rpart + Gr, Gr, red, &bcast);
+        const double rn = sqrt(rr);
+        if (c.reduce == kReduceSetup) {
+            const double bb = all_sum(rpart + 2 * Gr, Gr, red, &bcast);
+            c.bnorm = sqrt(bb);
+            c.norm0 = rn;
+            c.eps = eps_from(P, c.bnorm, rn);
+            c.iter = 0;
+            c.outer = 0;
+            if (!isfinite(bb) && c.status == 0) c.status = CEL_ERR_NONFINITE;
+            if (blockIdx.x == 0 && threadIdx.x == 0 && P.hist_cap > 0) P.hist[0] = rn;
+        } else {
+            c.outer = c.outer + 1;
+        }
+        c.alpha = rz;
+        c.norm2 = rn;
+        c.rnorm_ref = rn;
+        c.reduce = kReduceNone;
+        c.run = (c.status == 0 && c.iter < P.iter_max && alpha_ok(P, c.alpha) &&
+                 c.norm2 > c.eps)
+                    ? 1
+                    : 0;
+    }
+    if (!c.run) {
+        write_ctrl(cout, c);
+        return;
+    }
     __shared__ CsrStage<VT> sm;
     __shared__ int smap[kBlock];
-    __shared__ double red[kWaves];
     const int lb = logical_block();
-    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
-    int c0, c1;
-    chunk_range(nchunks, gridDim.x, lb, c0, c1);
-    for (int c = c0; c < c1; ++c) {
-        double acc = 0.0;
-        const int t1 = min((c + 1) * chunk_tiles, ntiles);
-        for (int t = c * chunk_tiles; t < t1; ++t) {
-            int row;
-            VT s = tile_rows<VT, VT, FMT, HALO>(A, t, p, sm, smap, row);
-            if (row < A.n) {
-                v[row] = s;
-                double prod = double(s) * double(p[row]);
-                acc = acc + prod;
-            }
+    const long long ntiles = (A.n + kTileRows - 1) / kTileRows;
+    long long t0, t1;
+    unit_range(ntiles, gridDim.x, lb, t0, t1);
+    double acc = 0.0;
+    for (long long t = t0; t < t1; ++t) {
+        int row;
+        VT s = tile_rows<VT, VT, FMT, HALO>(A, (int)t, p, sm, smap, row);
+        if (row < A.n) {
+            v[row] = s;
+            double prod = double(s) * double(p[row]);
+            acc = acc + prod;
         }
-        const double bs = block_sum(acc, red);
-        if (threadIdx.x == 0) part[c] = bs;
     }
-    if (!last_block(&st->ctr[1])) return;
-    const double vp = sum_partials(part, nchunks, red);
-    if (threadIdx.x == 0) {
-        const double sigma = st->alpha / vp;
-        st->vp = vp;
-        st->sigma = sigma;
-        if (!(vp > 0.0) || !isfinite(sigma)) {
-            st->status = CEL_ERR_BREAKDOWN;
-            st->stop = 1;
-        }
+    const double bs = block_sum(acc, red);
+    if (threadIdx.x == 0) part[lb] = bs;
+    write_ctrl(cout, c);
+}
+
+// 16-byte vectors for the streaming kernels
+template <typename VT>
+struct Vec;
+template <>
+struct Vec<double> {
+    using T = double2;
+    static constexpr int W = 2;
+};
+template <>
+struct Vec<float> {
+    using T = float4;
+    static constexpr int W = 4;
+};
+
+template <typename VT>
+__device__ __forceinline__ VT lane(const typename Vec<VT>::T& a, int k)
+{
+    if constexpr (Vec<VT>::W == 2) {
+        return k == 0 ? a.x : a.y;
+    } else {
+        return k == 0 ? a.x : k == 1 ? a.y : k == 2 ? a.z : a.w;
     }
 }
 
-// K2: r = r + (-sigma) v ; z = jac*r ; partials r.z, r.r ; last block:
-// beta = rz/alpha, alpha = rz, norm2 = sqrt(rr), k++, stop / replacement flag.
-template <typename VT, bool MIXED>
+template <typename VT>
+__device__ __forceinline__ void set_lane(typename Vec<VT>::T& a, int k, VT v)
+{
+    if constexpr (Vec<VT>::W == 2) {
+        if (k == 0) a.x = v;
+        else a.y = v;
+    } else {
+        if (k == 0) a.x = v;
+        else if (k == 1) a.y = v;
+        else if (k == 2) a.z = v;
+        else a.w = v;
+    }
+}
+
+// K2: sigma = alpha / sum(v.p) ; r = r + (-sigma) v ; z = jac*r ;
+// per-block partials of r.z and r.r.
+template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_r(long long n, VT* __restrict__ r,
                                                          const VT* __restrict__ v,
                                                          const VT* __restrict__ jac,
                                                          double* __restrict__ part,
-                                                         PcgState* __restrict__ st, int nchunks,
-                                                         int chunk_rows)
+                                                         const double* __restrict__ vpart, int G1,
+                                                         const Ctrl* __restrict__ cin,
+                                                         Ctrl* __restrict__ cout)
 {
-    if (st->stop) return;
     __shared__ double red[2 * kWaves];
+    __shared__ double bcast;
+    Ctrl c = read_ctrl(cin);
+    if (!c.run) {
+        write_ctrl(cout, c);
+        return;
+    }
+    const double vp = all_sum(vpart, G1, red, &bcast);
+    const double sigma = c.alpha / vp;
+    if (!(vp > 0.0) || !isfinite(sigma)) {
+        c.status = CEL_ERR_BREAKDOWN;
+        c.run = 0;
+        write_ctrl(cout, c);
+        return;
+    }
+    c.sigma = sigma;
+    using V = typename Vec<VT>::T;
+    constexpr int W = Vec<VT>::W;
     const int lb = logical_block();
-    const VT ms = VT(-st->sigma);
-    int c0, c1;
-    chunk_range(nchunks, gridDim.x, lb, c0, c1);
-    for (int c = c0; c < c1; ++c) {
-        double arz = 0.0, arr = 0.0;
-        const long long hi = min((long long)(c + 1) * chunk_rows, n);
-        for (long long i = (long long)c * chunk_rows + threadIdx.x; i < hi; i += kBlock) {
-            VT t = ms * v[i];
-            VT ri = r[i] + t;
-            r[i] = ri;
-            VT zi = ri * jac[i];
-            double rz = double(ri) * double(zi);
-            arz = arz + rz;
-            double rr = double(ri) * double(ri);
-            arr = arr + rr;
-        }
-        block_sum2(arz, arr, red);
-        if (threadIdx.x == 0) {
-            part[c] = arz;
-            part[nchunks + c] = arr;
-        }
-    }
-    if (!last_block(&st->ctr[2])) return;
-    const double rz = sum_partials(part, nchunks, red);
-    const double rr = sum_partials(part + nchunks, nchunks, red);
-    if (threadIdx.x == 0) {
-        const double beta = rz / st->alpha;
-        st->beta = beta;
-        st->alpha = rz;
-        st->norm2 = sqrt(rr);
-        st->iter = st->iter + 1;
-        if (st->iter < st->hist_cap) st->hist[st->iter] = st->norm2;
-        st->pending_xp = 1;
-        if (!MIXED) {
-            if (!loop_continues(st)) st->stop = 1;
+    const int G = gridDim.x;
+    const long long unit = (long long)kBlock * W;
+    long long u0, u1;
+    unit_range((n + unit - 1) / unit, G, lb, u0, u1);
+    const long long lo = u0 * unit, hi = min(u1 * unit, n);
+    const VT ms = VT(-sigma);
+    double arz = 0.0, arr = 0.0;
+    for (long long i = lo + (long long)threadIdx.x * W; i < hi; i += unit) {
+        if (i + W <= n) {
+            const V vv = *reinterpret_cast<const V*>(v + i);
+            V rv = *reinterpret_cast<const V*>(r + i);
+            const V jv = *reinterpret_cast<const V*>(jac + i);
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                VT t = ms * lane<VT>(vv, k);
+                VT ri = lane<VT>(rv, k) + t;
+                set_lane<VT>(rv, k, ri);
+                VT zi = ri * lane<VT>(jv, k);
+                double rz = double(ri) * double(zi);
+                arz = arz + rz;
+                double rr = double(ri) * double(ri);
+                arr = arr + rr;
+            }
+            *reinterpret_cast<V*>(r + i) = rv;
         } else {
-            // replace the FP32 residual by the FP64 one when it has dropped
-            // by ru_delta since the last replacement, looks converged, or the
-            // iteration budget is spent; the stop decision uses the FP64 norm
-            const bool drop = st->norm2 < st->ru_delta * st->rnorm_ref;
-            if (drop || st->norm2 <= st->eps || st->iter >= st->iter_max ||
-                !(fabs(st->alpha) > st->small) || !isfinite(st->norm2))
-                st->ru_flag = 1;
+            for (long long j = i; j < hi; ++j) {
+                VT t = ms * v[j];
+                VT ri = r[j] + t;
+                r[j] = ri;
+                VT zi = ri * jac[j];
+                double rz = double(ri) * double(zi);
+                arz = arz + rz;
+                double rr = double(ri) * double(ri);
+                arr = arr + rr;
+            }
         }
     }
+    block_sum2(arz, arr, red);
+    if (threadIdx.x == 0) {
+        part[lb] = arz;
+        part[G + lb] = arr;
+    }
+    write_ctrl(cout, c);
 }
 
-// K3: x = x + sigma p ; p = beta p + jac*r ; last block clears pending_xp.
-template <typename VT, typename XT>
+// K3: beta = sum(r.z)/alpha ; x = x + sigma p ; p = beta p + jac*r ;
+// alpha = r.z, norm2, k++, history, loop decision (FP64) or replacement
+// request (mixed).
+template <typename VT, typename XT, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_xp(long long n, XT* __restrict__ x,
                                                           VT* __restrict__ p,
                                                           const VT* __restrict__ r,
                                                           const VT* __restrict__ jac,
-                                                          PcgState* __restrict__ st, int nchunks,
-                                                          int chunk_rows)
+                                                          const double* __restrict__ rpart, int G2,
+                                                          const Ctrl* __restrict__ cin,
+                                                          Ctrl* __restrict__ cout, Prm P)
 {
-    if (!st->pending_xp) return;
-    const int lb = logical_block();
-    const VT sigma = VT(st->sigma), beta = VT(st->beta);
-    int c0, c1;
-    chunk_range(nchunks, gridDim.x, lb, c0, c1);
-    const long long lo = (long long)c0 * chunk_rows;
-    const long long hi = min((long long)c1 * chunk_rows, n);
-    for (long long i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const VT pi = p[i];
-        XT t = XT(sigma * pi);
-        x[i] = x[i] + t;
-        VT zi = r[i] * jac[i];
-        VT bp = beta * pi;
-        p[i] = bp + zi;
+    __shared__ double red[kWaves];
+    __shared__ double bcast;
+    Ctrl c = read_ctrl(cin);
+    if (!c.run) {
+        write_ctrl(cout, c);
+        return;
     }
-    if (last_block(&st->ctr[3]) && threadIdx.x == 0) st->pending_xp = 0;
+    const double rz = all_sum(rpart, G2, red, &bcast);
+    const double rr = all_sum(rpart + G2, G2, red, &bcast);
+    const double beta_d = rz / c.alpha;
+    using V = typename Vec<VT>::T;
+    constexpr int W = Vec<VT>::W;
+    const int lb = logical_block();
+    const long long unit = (long long)kBlock * W;
+    long long u0, u1;
+    unit_range((n + unit - 1) / unit, gridDim.x, lb, u0, u1);
+    const long long lo = u0 * unit, hi = min(u1 * unit, n);
+    const VT sigma = VT(c.sigma), beta = VT(beta_d);
+    for (long long i = lo + (long long)threadIdx.x * W; i < hi; i += unit) {
+        if (i + W <= n) {
+            V pv = *reinterpret_cast<const V*>(p + i);
+            const V rv = *reinterpret_cast<const V*>(r + i);
+            const V jv = *reinterpret_cast<const V*>(jac + i);
+            XT xs[W];
+#pragma unroll
+            for (int k = 0; k < W; ++k) xs[k] = x[i + k];
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                const VT pi = lane<VT>(pv, k);
+                XT t = XT(sigma * pi);
+                xs[k] = xs[k] + t;
+                VT zi = lane<VT>(rv, k) * lane<VT>(jv, k);
+                VT bp = beta * pi;
+                set_lane<VT>(pv, k, bp + zi);
+            }
+#pragma unroll
+            for (int k = 0; k < W; ++k) x[i + k] = xs[k];
+            *reinterpret_cast<V*>(p + i) = pv;
+        } else {
+            for (long long j = i; j < hi; ++j) {
+                const VT pi = p[j];
+                XT t = XT(sigma * pi);
+                x[j] = x[j] + t;
+                VT zi = r[j] * jac[j];
+                VT bp = beta * pi;
+                p[j] = bp + zi;
+            }
+        }
+    }
+    c.beta = beta_d;
+    c.alpha = rz;
+    c.norm2 = sqrt(rr);
+    c.iter = c.iter + 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && c.iter < P.hist_cap) P.hist[c.iter] = c.norm2;
+    if (!MIXED) {
+        c.run = (c.iter < P.iter_max && alpha_ok(P, c.alpha) && c.norm2 > c.eps) ? 1 : 0;
+    } else {
+        // replace the FP32 residual by the FP64 one when it has dropped by
+        // ru_delta since the last replacement, looks converged, or the
+        // budget is spent; the stop decision is taken on the FP64 norm
+        const bool drop = c.norm2 < P.ru_delta * c.rnorm_ref;
+        c.ru = (drop || c.norm2 <= c.eps || c.iter >= P.iter_max || !alpha_ok(P, c.alpha) ||
+                !isfinite(c.norm2))
+                   ? 1
+                   : 0;
+    }
+    write_ctrl(cout, c);
 }
 
 // Mixed form, replacement step 1: x64 = x64 + e32 ; e32 = 0.
 __global__ __launch_bounds__(kBlock) void k_ru_flush(long long n, double* __restrict__ x,
                                                      float* __restrict__ e,
-                                                     const PcgState* __restrict__ st, int force,
-                                                     int nchunks, int chunk_rows)
+                                                     const Ctrl* __restrict__ cin,
+                                                     Ctrl* __restrict__ cout, int force)
 {
-    if (!st->ru_flag && !force) return;
-    const int lb = logical_block();
-    int c0, c1;
-    chunk_range(nchunks, gridDim.x, lb, c0, c1);
-    const long long lo = (long long)c0 * chunk_rows;
-    const long long hi = min((long long)c1 * chunk_rows, n);
-    for (long long i = lo + threadIdx.x; i < hi; i += kBlock) {
+    const Ctrl c = read_ctrl(cin);
+    write_ctrl(cout, c);
+    if (!force && (!c.run || !c.ru)) return;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
         double t = (double)e[i];
         x[i] = x[i] + t;
         e[i] = 0.0f;
     }
 }
 
-// Mixed form, replacement step 2: r32 = float(b - A64 x64), z = jac32 r32,
-// FP64 norm of the FP64 residual; last block decides convergence on it.
-template <int FMT>
-__global__ __launch_bounds__(kBlock) void k_ru_residual(MatView A, const double* __restrict__ x,
-                                                        const double* __restrict__ b,
-                                                        const float* __restrict__ jac32,
-                                                        float* __restrict__ r, float* __restrict__ p,
-                                                        double* __restrict__ part,
-                                                        PcgState* __restrict__ st, int nchunks,
-                                                        int chunk_tiles)
-{
-    if (!st->ru_flag) return;
-    __shared__ CsrStage<double> sm;
-    __shared__ int smap[kBlock];
-    __shared__ double red[2 * kWaves];
-    const int lb = logical_block();
-    const int restart = st->ru_restart;
-    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
-    int c0, c1;
-    chunk_range(nchunks, gridDim.x, lb, c0, c1);
-    for (int c = c0; c < c1; ++c) {
-        double arz = 0.0, arr = 0.0;
-        const int t1 = min((c + 1) * chunk_tiles, ntiles);
-        for (int t = c * chunk_tiles; t < t1; ++t) {
-            int row;
-            double s = tile_rows<double, double, FMT, false>(A, t, x, sm, smap, row);
-            if (row < A.n) {
-                const double ri = b[row] - s;
-                double rr = ri * ri;
-                arr = arr + rr;
-                const float rf = (float)ri;
-                const float zf = rf * jac32[row];
-                r[row] = rf;
-                if (restart) p[row] = zf;
-                double rz = (double)rf * (double)zf;
-                arz = arz + rz;
-            }
-        }
-        block_sum2(arz, arr, red);
-        if (threadIdx.x == 0) {
-            part[c] = arz;
-            part[nchunks + c] = arr;
-        }
-    }
-    if (!last_block(&st->ctr[0])) return;
-    const double rz = sum_partials(part, nchunks, red);
-    const double rr = sum_partials(part + nchunks, nchunks, red);
-    if (threadIdx.x == 0) {
-        const double s = sqrt(rr);
-        st->alpha = rz;
-        st->norm2 = s;
-        st->rnorm_ref = s;
-        st->outer = st->outer + 1;
-        st->ru_flag = 0;
-        if (!(s > st->eps) || st->iter >= st->iter_max || !(fabs(rz) > st->small) ||
-            !isfinite(s))
-            st->stop = 1;
-    }
-}
-
 // ------------------------------------------------------------------ host side
 struct Workspace {
-    DevBuf<double> r, p, v, x, b, part, ghost, sendbuf, hist;
+    DevBuf<double> r, p, v, x, b, part1, part2, rpart, ghost, sendbuf, hist;
     DevBuf<float> r32, p32, v32, e32;
-    DevBuf<PcgState> st;
+    DevBuf<Ctrl> ctrl;
     DevBuf<int> hrow_tile;
 };
 
@@ -435,8 +526,6 @@ static Workspace& ws_of(Mat& A)
     if (!A.ws) A.ws = new Workspace();
     return *A.ws;
 }
-
-
 
 // Compact halo rows per 256-row tile: hrow_tile[t] = first compact row >= t*256.
 static int build_hrow_tile(const Mat& A, Workspace& W, int ntiles)
@@ -456,10 +545,15 @@ static int build_hrow_tile(const Mat& A, Workspace& W, int ntiles)
     return CEL_OK;
 }
 
-struct Launch {
-    int ntiles, Gt, Gv, Gt_res, Gru;
-    int chunk_rows, chunk_tiles, nchunks;
-};
+// Resident blocks of several kernels at once: the smallest, so every
+// variant that feeds the same consumer uses the same grid (and therefore
+// the same partials and the same sums).
+static int common_grid(std::initializer_list<const void*> kernels, long long work)
+{
+    int g = 1 << 30;
+    for (const void* k : kernels) g = std::min(g, resident_grid(k, work));
+    return g;
+}
 
 int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel_pcg_stats& stats,
               double* hist)
@@ -480,37 +574,36 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     if (prm.iter_max < 0) return fail(CEL_ERR_ARG, "iter_max < 0");
 
     Workspace& W = ws_of(A);
-    Launch L;
-    L.ntiles = (n + kTileRows - 1) / kTileRows;
-    if (L.ntiles == 0) L.ntiles = 1;
-    {
-        const void* k1;
-        const void* kres;
-        if (!mixed) {
-            k1 = halo ? (const void*)k_pcg_spmv<double, CEL_FMT_CSR, true>
-                      : fmt == CEL_FMT_CSR ? (const void*)k_pcg_spmv<double, CEL_FMT_CSR, false>
-                                           : (const void*)k_pcg_spmv<double, CEL_FMT_JAD, false>;
-            kres = halo ? (const void*)k_residual<double, CEL_FMT_CSR, true>
-                        : fmt == CEL_FMT_CSR ? (const void*)k_residual<double, CEL_FMT_CSR, false>
-                                             : (const void*)k_residual<double, CEL_FMT_JAD, false>;
+    const int ntiles = std::max(1, (n + kTileRows - 1) / kTileRows);
+    // grids: K1 variants that must agree (with and without halo) share one
+    // grid; the residual kernels share one; K2 and K3 share one.
+    int G1, Gr, Gv;
+    if (!mixed) {
+        if (fmt == CEL_FMT_CSR) {
+            G1 = common_grid({(const void*)k_pcg_spmv<double, CEL_FMT_CSR, false>,
+                              (const void*)k_pcg_spmv<double, CEL_FMT_CSR, true>}, ntiles);
+            Gr = common_grid({(const void*)k_residual<double, CEL_FMT_CSR, false, false>,
+                              (const void*)k_residual<double, CEL_FMT_CSR, true, false>}, ntiles);
         } else {
-            k1 = fmt == CEL_FMT_CSR ? (const void*)k_pcg_spmv<float, CEL_FMT_CSR, false>
-                                    : (const void*)k_pcg_spmv<float, CEL_FMT_JAD, false>;
-            kres = fmt == CEL_FMT_CSR ? (const void*)k_residual<float, CEL_FMT_CSR, false>
-                                      : (const void*)k_residual<float, CEL_FMT_JAD, false>;
+            G1 = common_grid({(const void*)k_pcg_spmv<double, CEL_FMT_JAD, false>}, ntiles);
+            Gr = common_grid({(const void*)k_residual<double, CEL_FMT_JAD, false, false>}, ntiles);
         }
-        const void* kru = fmt == CEL_FMT_CSR ? (const void*)k_ru_residual<CEL_FMT_CSR>
-                                             : (const void*)k_ru_residual<CEL_FMT_JAD>;
-        L.chunk_rows = chunk_rows_for(n);
-        L.chunk_tiles = L.chunk_rows / kTileRows;
-        L.nchunks = (int)(((long long)n + L.chunk_rows - 1) / L.chunk_rows);
-        if (L.nchunks < 1) L.nchunks = 1;
-        L.Gt = resident_grid(k1, L.nchunks);
-        L.Gt_res = resident_grid(kres, L.nchunks);
-        L.Gru = resident_grid(kru, L.nchunks);
-        L.Gv = resident_grid((const void*)k_pcg_update_r<double, false>, L.nchunks);
+    } else {
+        if (fmt == CEL_FMT_CSR) {
+            G1 = common_grid({(const void*)k_pcg_spmv<float, CEL_FMT_CSR, false>}, ntiles);
+            Gr = common_grid({(const void*)k_residual<float, CEL_FMT_CSR, false, false>,
+                              (const void*)k_residual<float, CEL_FMT_CSR, false, true>}, ntiles);
+        } else {
+            G1 = common_grid({(const void*)k_pcg_spmv<float, CEL_FMT_JAD, false>}, ntiles);
+            Gr = common_grid({(const void*)k_residual<float, CEL_FMT_JAD, false, false>,
+                              (const void*)k_residual<float, CEL_FMT_JAD, false, true>}, ntiles);
+        }
     }
-    const int Gmax = L.nchunks;
+    {
+        const long long units = ((long long)n + 2 * kBlock - 1) / (2 * kBlock);
+        Gv = (int)std::min<long long>(rt().num_cus * 4, std::max<long long>(1, units));
+        if (Gv >= kXcds) Gv = Gv / kXcds * kXcds;
+    }
 
     // matrix data for the mixed inner operator (once per matrix)
     if (mixed) {
@@ -526,8 +619,10 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
 
     // workspace
     const size_t nn = n > 0 ? n : 1;
-    CEL_TRY(W.part.alloc(3 * (size_t)Gmax));
-    CEL_TRY(W.st.alloc(1));
+    CEL_TRY(W.part1.alloc((size_t)G1));
+    CEL_TRY(W.part2.alloc(2 * (size_t)Gv));
+    CEL_TRY(W.rpart.alloc(3 * (size_t)Gr));
+    CEL_TRY(W.ctrl.alloc(5));
     if (mixed) {
         CEL_TRY(W.r32.alloc(nn));
         CEL_TRY(W.p32.alloc(nn));
@@ -555,19 +650,29 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         CEL_TRY(W.ghost.alloc(A.ghost_len > 0 ? A.ghost_len : 1));
         CEL_TRY(W.sendbuf.alloc(A.send_total > 0 ? A.send_total : 1));
         hipLaunchKernelGGL(k_fill, dim3(64), dim3(kBlock), 0, s, (long long)W.ghost.n, W.ghost.p, 0.0);
-        CEL_TRY(build_hrow_tile(A, W, L.ntiles));
+        CEL_TRY(build_hrow_tile(A, W, ntiles));
     }
 
-    PcgState h{};
-    h.tol = prm.tol;
-    h.tol_mode = prm.tol_mode;
-    h.iter_max = prm.iter_max;
-    h.small = 1.1 * 2.220446049250313e-16;  // cel_real_small (cel_types_module.f90:33)
-    h.ru_delta = prm.ru_delta > 0 ? prm.ru_delta : 0.1;
-    h.ru_restart = prm.ru_restart;
-    h.hist = hist_cap ? W.hist.p : nullptr;
-    h.hist_cap = hist_cap;
-    CEL_HIP(hipMemcpyAsync(W.st.p, &h, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    Prm P{};
+    P.tol = prm.tol;
+    P.tol_mode = prm.tol_mode;
+    P.iter_max = prm.iter_max;
+    P.small = 1.1 * 2.220446049250313e-16;  // cel_real_small (cel_types_module.f90:33)
+    P.ru_delta = prm.ru_delta > 0 ? prm.ru_delta : 0.1;
+    P.ru_restart = prm.ru_restart;
+    P.alpha_stop = prm.alpha_stop;
+    P.hist = hist_cap ? W.hist.p : nullptr;
+    P.hist_cap = hist_cap;
+    Ctrl c0{};
+    c0.run = 1;
+    CEL_HIP(hipMemcpyAsync(W.ctrl.p, &c0, sizeof(Ctrl), hipMemcpyHostToDevice, s));
+    // token slots: 0 -> K1 -> 1 -> K2 -> 2 -> K3 -> 0 (FP64);
+    // mixed: K3 -> 3 -> flush -> 4 -> replacement residual -> 0
+    Ctrl* C0 = W.ctrl.p;
+    Ctrl* C1 = W.ctrl.p + 1;
+    Ctrl* C2 = W.ctrl.p + 2;
+    Ctrl* C3 = W.ctrl.p + 3;
+    Ctrl* C4 = W.ctrl.p + 4;
 
     MatView V{};
     V.n = n;
@@ -598,30 +703,30 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
 
     // x0 (cel_cgalg_module.f90:209 uses 1.0)
     if (prm.x0_mode != CEL_X0_USER)
-        hipLaunchKernelGGL(k_fill, dim3(std::max(1, std::min(L.Gv, 4096))), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL(k_fill, dim3(std::max(1, std::min(Gv * 2, 4096))), dim3(kBlock), 0, s,
                            (long long)n, dx, prm.x0_mode == CEL_X0_ONE ? 1.0 : 0.0);
     if (halo) CEL_TRY(launch_face_pack(A, dx, W.sendbuf.p));
 
-    // r0, p0, alpha, norm2
-#define RES_ARGS(OT, R, P)                                                                   \
-    V, dx, db, A.jac.p, A.jac32.p, (OT*)(R), (OT*)(P), W.e32.p, W.part.p, W.st.p, L.nchunks, \
-        L.chunk_tiles
+    // r0, p0 and their partials; the first K1 folds them in.  The set-up
+    // reads and writes token slot 0 (only block 0 writes, after its read).
+#define RES_ARGS(OT, R, P_)                                                                \
+    V, dx, db, A.jac.p, A.jac32.p, (OT*)(R), (OT*)(P_), W.e32.p, W.rpart.p, C0, C0, P
     if (!mixed) {
         if (fmt == CEL_FMT_CSR && halo)
-            hipLaunchKernelGGL((k_residual<double, CEL_FMT_CSR, true>), dim3(L.Gt_res), dim3(kBlock), 0, s,
+            hipLaunchKernelGGL((k_residual<double, CEL_FMT_CSR, true, false>), dim3(Gr), dim3(kBlock), 0, s,
                                RES_ARGS(double, W.r.p, W.p.p));
         else if (fmt == CEL_FMT_CSR)
-            hipLaunchKernelGGL((k_residual<double, CEL_FMT_CSR, false>), dim3(L.Gt_res), dim3(kBlock), 0,
+            hipLaunchKernelGGL((k_residual<double, CEL_FMT_CSR, false, false>), dim3(Gr), dim3(kBlock), 0,
                                s, RES_ARGS(double, W.r.p, W.p.p));
         else
-            hipLaunchKernelGGL((k_residual<double, CEL_FMT_JAD, false>), dim3(L.Gt_res), dim3(kBlock), 0,
+            hipLaunchKernelGGL((k_residual<double, CEL_FMT_JAD, false, false>), dim3(Gr), dim3(kBlock), 0,
                                s, RES_ARGS(double, W.r.p, W.p.p));
     } else {
         if (fmt == CEL_FMT_CSR)
-            hipLaunchKernelGGL((k_residual<float, CEL_FMT_CSR, false>), dim3(L.Gt_res), dim3(kBlock), 0, s,
+            hipLaunchKernelGGL((k_residual<float, CEL_FMT_CSR, false, false>), dim3(Gr), dim3(kBlock), 0, s,
                                RES_ARGS(float, W.r32.p, W.p32.p));
         else
-            hipLaunchKernelGGL((k_residual<float, CEL_FMT_JAD, false>), dim3(L.Gt_res), dim3(kBlock), 0, s,
+            hipLaunchKernelGGL((k_residual<float, CEL_FMT_JAD, false, false>), dim3(Gr), dim3(kBlock), 0, s,
                                RES_ARGS(float, W.r32.p, W.p32.p));
     }
 #undef RES_ARGS
@@ -640,8 +745,10 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         return pev[pev_used++];
     };
 
+    const long long n_ll = n;
     auto enqueue_iter = [&](bool profile) -> int {
         hipEvent_t ea = nullptr, eb = nullptr;
+        if (halo) CEL_TRY(launch_face_pack(A, W.p.p, W.sendbuf.p));
         if (profile) {
             ea = prof_event();
             eb = prof_event();
@@ -649,56 +756,53 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
             CEL_HIP(hipEventRecord(ea, s));
         }
         if (!mixed) {
-            if (halo) {
-                CEL_TRY(launch_face_pack(A, W.p.p, W.sendbuf.p));
-                hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_CSR, true>), dim3(L.Gt), dim3(kBlock), 0, s,
-                                   V, W.p.p, W.v.p, W.part.p, W.st.p, L.nchunks, L.chunk_tiles);
-            } else if (fmt == CEL_FMT_CSR) {
-                hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_CSR, false>), dim3(L.Gt), dim3(kBlock), 0, s,
-                                   V, W.p.p, W.v.p, W.part.p, W.st.p, L.nchunks, L.chunk_tiles);
-            } else {
-                hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_JAD, false>), dim3(L.Gt), dim3(kBlock), 0, s,
-                                   V, W.p.p, W.v.p, W.part.p, W.st.p, L.nchunks, L.chunk_tiles);
-            }
+            if (halo)
+                hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_CSR, true>), dim3(G1), dim3(kBlock), 0, s, V,
+                                   W.p.p, W.v.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
+            else if (fmt == CEL_FMT_CSR)
+                hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_CSR, false>), dim3(G1), dim3(kBlock), 0, s, V,
+                                   W.p.p, W.v.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
+            else
+                hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_JAD, false>), dim3(G1), dim3(kBlock), 0, s, V,
+                                   W.p.p, W.v.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
             if (profile) CEL_HIP(hipEventRecord(eb, s));
-            hipLaunchKernelGGL((k_pcg_update_r<double, false>), dim3(L.Gv), dim3(kBlock), 0, s, (long long)n,
-                               W.r.p, W.v.p, A.jac.p, W.part.p, W.st.p, L.nchunks, L.chunk_rows);
-            hipLaunchKernelGGL((k_pcg_update_xp<double, double>), dim3(L.Gv), dim3(kBlock), 0, s,
-                               (long long)n, dx, W.p.p, W.r.p, A.jac.p, W.st.p, L.nchunks, L.chunk_rows);
+            hipLaunchKernelGGL((k_pcg_update_r<double>), dim3(Gv), dim3(kBlock), 0, s, n_ll, W.r.p, W.v.p,
+                               A.jac.p, W.part2.p, W.part1.p, G1, C1, C2);
+            hipLaunchKernelGGL((k_pcg_update_xp<double, double, false>), dim3(Gv), dim3(kBlock), 0, s, n_ll,
+                               dx, W.p.p, W.r.p, A.jac.p, W.part2.p, Gv, C2, C0, P);
         } else {
             if (fmt == CEL_FMT_CSR)
-                hipLaunchKernelGGL((k_pcg_spmv<float, CEL_FMT_CSR, false>), dim3(L.Gt), dim3(kBlock), 0, s,
-                                   V, W.p32.p, W.v32.p, W.part.p, W.st.p, L.nchunks, L.chunk_tiles);
+                hipLaunchKernelGGL((k_pcg_spmv<float, CEL_FMT_CSR, false>), dim3(G1), dim3(kBlock), 0, s, V,
+                                   W.p32.p, W.v32.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
             else
-                hipLaunchKernelGGL((k_pcg_spmv<float, CEL_FMT_JAD, false>), dim3(L.Gt), dim3(kBlock), 0, s,
-                                   V, W.p32.p, W.v32.p, W.part.p, W.st.p, L.nchunks, L.chunk_tiles);
+                hipLaunchKernelGGL((k_pcg_spmv<float, CEL_FMT_JAD, false>), dim3(G1), dim3(kBlock), 0, s, V,
+                                   W.p32.p, W.v32.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
             if (profile) CEL_HIP(hipEventRecord(eb, s));
-            hipLaunchKernelGGL((k_pcg_update_r<float, true>), dim3(L.Gv), dim3(kBlock), 0, s, (long long)n,
-                               W.r32.p, W.v32.p, A.jac32.p, W.part.p, W.st.p, L.nchunks, L.chunk_rows);
-            hipLaunchKernelGGL((k_pcg_update_xp<float, float>), dim3(L.Gv), dim3(kBlock), 0, s,
-                               (long long)n, W.e32.p, W.p32.p, W.r32.p, A.jac32.p, W.st.p, L.nchunks,
-                               L.chunk_rows);
-            hipLaunchKernelGGL(k_ru_flush, dim3(L.Gv), dim3(kBlock), 0, s, (long long)n, dx, W.e32.p,
-                               W.st.p, 0, L.nchunks, L.chunk_rows);
+            hipLaunchKernelGGL((k_pcg_update_r<float>), dim3(Gv), dim3(kBlock), 0, s, n_ll, W.r32.p, W.v32.p,
+                               A.jac32.p, W.part2.p, W.part1.p, G1, C1, C2);
+            hipLaunchKernelGGL((k_pcg_update_xp<float, float, true>), dim3(Gv), dim3(kBlock), 0, s, n_ll,
+                               W.e32.p, W.p32.p, W.r32.p, A.jac32.p, W.part2.p, Gv, C2, C3, P);
+            hipLaunchKernelGGL(k_ru_flush, dim3(Gv), dim3(kBlock), 0, s, n_ll, dx, W.e32.p, C3, C4, 0);
             if (fmt == CEL_FMT_CSR)
-                hipLaunchKernelGGL(k_ru_residual<CEL_FMT_CSR>, dim3(L.Gru), dim3(kBlock), 0, s, V, dx,
-                                   db, A.jac32.p, W.r32.p, W.p32.p, W.part.p, W.st.p, L.nchunks,
-                                   L.chunk_tiles);
+                hipLaunchKernelGGL((k_residual<float, CEL_FMT_CSR, false, true>), dim3(Gr), dim3(kBlock), 0, s,
+                                   V, dx, db, A.jac.p, A.jac32.p, W.r32.p, W.p32.p, W.e32.p, W.rpart.p, C4,
+                                   C0, P);
             else
-                hipLaunchKernelGGL(k_ru_residual<CEL_FMT_JAD>, dim3(L.Gru), dim3(kBlock), 0, s, V, dx,
-                                   db, A.jac32.p, W.r32.p, W.p32.p, W.part.p, W.st.p, L.nchunks,
-                                   L.chunk_tiles);
+                hipLaunchKernelGGL((k_residual<float, CEL_FMT_JAD, false, true>), dim3(Gr), dim3(kBlock), 0, s,
+                                   V, dx, db, A.jac.p, A.jac32.p, W.r32.p, W.p32.p, W.e32.p, W.rpart.p, C4,
+                                   C0, P);
         }
         CEL_HIP(hipGetLastError());
         return CEL_OK;
     };
 
     // iteration loop: chunks of `chunk` iterations, one chunk in flight
-    // ahead of the host's check of the stop flag
+    // ahead of the host's check of the token.  The loop needs at most
+    // iter_max iterations plus one K1 that folds in the final partials.
     const int chunk = prm.check_every > 0 ? prm.check_every : 16;
-    int* hstop = nullptr;
-    CEL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hstop), 2 * sizeof(int), hipHostMallocDefault));
-    hstop[0] = hstop[1] = 0;
+    int* hrun = nullptr;
+    CEL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hrun), 2 * sizeof(int), hipHostMallocDefault));
+    hrun[0] = hrun[1] = 1;
     hipEvent_t cev[2];
     CEL_HIP(hipEventCreateWithFlags(&cev[0], hipEventDisableTiming));
     CEL_HIP(hipEventCreateWithFlags(&cev[1], hipEventDisableTiming));
@@ -707,11 +811,9 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     hipGraph_t graph = nullptr;
     const bool use_graph = prm.use_graph && !prm.profile;
     int status = CEL_OK;
+    const long long budget = (long long)prm.iter_max + 1;
     long long enqueued = 0;
     int k = 0;
-    // the mixed form may need iterations beyond iter_max only for the final
-    // replacement, which runs inside the same chunk
-    const long long budget = prm.iter_max;
     while (enqueued < budget) {
         const int c = (int)std::min<long long>(chunk, budget - enqueued);
         if (use_graph && c == chunk) {
@@ -736,18 +838,17 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
             if (status < 0) break;
         }
         enqueued += c;
-        CEL_HIP(hipMemcpyAsync(&hstop[k & 1], &W.st.p->stop, sizeof(int), hipMemcpyDeviceToHost, s));
+        CEL_HIP(hipMemcpyAsync(&hrun[k & 1], &C0->run, sizeof(int), hipMemcpyDeviceToHost, s));
         CEL_HIP(hipEventRecord(cev[k & 1], s));
         if (k > 0) {
             CEL_HIP(hipEventSynchronize(cev[(k - 1) & 1]));
-            if (hstop[(k - 1) & 1]) break;
+            if (!hrun[(k - 1) & 1]) break;
         }
         ++k;
     }
     if (mixed) {
         // a budget that ran out mid-way leaves a correction in e: fold it in
-        hipLaunchKernelGGL(k_ru_flush, dim3(L.Gv), dim3(kBlock), 0, s, (long long)n, dx, W.e32.p, W.st.p, 1,
-                           L.nchunks, L.chunk_rows);
+        hipLaunchKernelGGL(k_ru_flush, dim3(Gv), dim3(kBlock), 0, s, n_ll, dx, W.e32.p, C0, C3, 1);
     }
     CEL_HIP(hipEventRecord(e2, s));
     CEL_HIP(hipStreamSynchronize(s));
@@ -755,11 +856,11 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     if (graph) (void)hipGraphDestroy(graph);
     (void)hipEventDestroy(cev[0]);
     (void)hipEventDestroy(cev[1]);
-    (void)hipHostFree(hstop);
+    (void)hipHostFree(hrun);
     if (status < 0) return status;
 
-    PcgState out;
-    CEL_HIP(hipMemcpy(&out, W.st.p, sizeof(PcgState), hipMemcpyDeviceToHost));
+    Ctrl out;
+    CEL_HIP(hipMemcpy(&out, C0, sizeof(Ctrl), hipMemcpyDeviceToHost));
     float ms_setup = 0, ms_all = 0;
     CEL_HIP(hipEventElapsedTime(&ms_setup, e0, e1));
     CEL_HIP(hipEventElapsedTime(&ms_all, e0, e2));
@@ -811,7 +912,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     }
     const bool converged = out.norm2 <= out.eps;
     stats.status = converged ? CEL_OK : CEL_NOT_CONVERGED;
-    if (!converged && out.iter < prm.iter_max && !(std::fabs(out.alpha) > h.small))
+    if (!converged && out.iter < prm.iter_max && prm.alpha_stop && !(std::fabs(out.alpha) > P.small))
         stats.status = CEL_OK;  // stopped on |alpha| <= 1.1 eps (reference condition)
     return stats.status;
 }

@@ -91,6 +91,8 @@ typedef struct {
     int check_every;   /* iterations per host poll of the device state (default 16) */
     int use_graph;     /* 1: replay one captured chunk of iterations as a hipGraph */
     int profile;       /* 1: time every SpMV launch with HIP events (stats) */
+    int alpha_stop;    /* 1 (reference): also stop when |alpha| <= 1.1 eps_mach, an absolute
+                          test (cel_cgalg_module.f90:248); 0: only the norm test */
 } cel_pcg_params;
 
 /* Per-solve results.  The reference returns iter_done and norm2
@@ -217,6 +219,15 @@ int cel_gpu_mat_distr_split(cel_gpu_mat** A, int nranks, const int* vtxdist, int
  * recv_perm (receive list, blocks >= 1, NULL to skip). */
 int cel_gpu_mat_get_block(const cel_gpu_mat* A, int b, int* proc, int* nnz, int* rows, int* cols,
                           double* vals, int* xadj, int* recv_len, int* recv_perm);
+/* Appends one halo block to a local block built by any constructor (e.g. a
+ * generated subdomain): the same per-rank block the split would create for
+ * owner `proc` (cel_sp_mat_distr_module.f90:117-144, :249).  rows are this
+ * rank's local rows, cols the owner's local rows (both index_base, row-
+ * sorted); proc_first_row is vtxdist(proc+1) (1-based), proc_len the owner's
+ * row count.  Blocks must be added in ascending rank order. */
+int cel_gpu_mat_add_halo_block(cel_gpu_mat* A, int proc, int proc_first_row, int proc_len,
+                               int nnz, const int* rows, const int* cols, const double* vals,
+                               int index_base);
 /* Send lists: the owner packs, for each peer, the entries that peer listed
  * (cel_comm_get_comm_buffer_connections, src/fcomm/cel_comm_module.f90:312-357).
  * perm is 1-based local indices. */
@@ -232,6 +243,11 @@ int cel_gpu_face_pack(cel_gpu_mat* A, const double* x, double* sendbuf, int mems
 int cel_gpu_spmv_distr(cel_gpu_mat* A, int format, const double* x, const double* ghost,
                        double* y, int memspace);
 int cel_gpu_mat_ghost_len(const cel_gpu_mat* A, int* ghost_len, int* send_total);
+
+/* Mean time of a streaming device copy of `bytes` (read + write), for the
+ * measured bandwidth ceiling next to the 8 TB/s nominal peak.  No reference
+ * counterpart. */
+int cel_gpu_stream_copy_time(size_t bytes, int reps, double* mean_ms);
 
 /* ---- Jacobi-PCG -------------------------------------------------------- */
 void cel_gpu_pcg_params_default(cel_pcg_params* p);
