@@ -50,13 +50,14 @@ struct Ctrl {
     int reduce;  // partials the next K1 must fold in first: 0 none, 1 set-up, 2 replacement
     int ru;      // mixed: replace the residual at the end of this iteration
     int outer;   // mixed: replacements done
-    int pad[2];
+    int since;   // mixed: iterations since the last replacement
+    int stall;   // mixed: consecutive replacements that did not halve the FP64 residual
 };
 
 // Constant solver parameters.
 struct Prm {
     double tol, small, ru_delta;
-    int tol_mode, iter_max, ru_restart, hist_cap, alpha_stop;
+    int tol_mode, iter_max, ru_restart, hist_cap, alpha_stop, ru_max_inner;
     double* hist;
 };
 
@@ -267,13 +268,17 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(MatView A, const VT* __rest
             if (blockIdx.x == 0 && threadIdx.x == 0 && P.hist_cap > 0) P.hist[0] = rn;
         } else {
             c.outer = c.outer + 1;
+            // attainable accuracy: two cycles in a row that fail to halve the
+            // FP64 residual end the solve (not converged)
+            c.stall = (rn > 0.5 * c.rnorm_ref) ? c.stall + 1 : 0;
         }
+        c.since = 0;
         c.alpha = rz;
         c.norm2 = rn;
         c.rnorm_ref = rn;
         c.reduce = kReduceNone;
         c.run = (c.status == 0 && c.iter < P.iter_max && alpha_ok(P, c.alpha) &&
-                 c.norm2 > c.eps)
+                 c.norm2 > c.eps && c.stall < 2)
                     ? 1
                     : 0;
     }
@@ -485,11 +490,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xp(long long n, XT* __res
         c.run = (c.iter < P.iter_max && alpha_ok(P, c.alpha) && c.norm2 > c.eps) ? 1 : 0;
     } else {
         // replace the FP32 residual by the FP64 one when it has dropped by
-        // ru_delta since the last replacement, looks converged, or the
-        // budget is spent; the stop decision is taken on the FP64 norm
+        // ru_delta since the last replacement, looks converged, the budget
+        // is spent, or the FP32 recurrence has stopped tracking the FP64
+        // residual (grown past it, or a cycle far longer than expected);
+        // the stop decision is taken on the FP64 norm
+        c.since = c.since + 1;
         const bool drop = c.norm2 < P.ru_delta * c.rnorm_ref;
-        c.ru = (drop || c.norm2 <= c.eps || c.iter >= P.iter_max || !alpha_ok(P, c.alpha) ||
-                !isfinite(c.norm2))
+        const bool astray = c.norm2 > 2.0 * c.rnorm_ref || c.since >= P.ru_max_inner;
+        c.ru = (drop || astray || c.norm2 <= c.eps || c.iter >= P.iter_max ||
+                !alpha_ok(P, c.alpha) || !isfinite(c.norm2))
                    ? 1
                    : 0;
     }
@@ -661,6 +670,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     P.ru_delta = prm.ru_delta > 0 ? prm.ru_delta : 0.1;
     P.ru_restart = prm.ru_restart;
     P.alpha_stop = prm.alpha_stop;
+    P.ru_max_inner = 1000;
     P.hist = hist_cap ? W.hist.p : nullptr;
     P.hist_cap = hist_cap;
     Ctrl c0{};

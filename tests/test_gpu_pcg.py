@@ -186,6 +186,25 @@ def test_mixed_restart_variant_converges(gpu):
     assert got["status"] == 0 and true_rel <= 1e-10
 
 
+def test_mixed_stops_at_attainable_accuracy(gpu):
+    """A tolerance below what FP64 residuals can show for this
+    variable-coefficient system ends as NOT_CONVERGED with a finite,
+    accurate x (two replacement cycles that fail to halve the FP64
+    residual), not as an FP32 overflow."""
+    n = 64
+    rng = np.random.default_rng(4)
+    k = np.exp(rng.normal(0, 1, n ** 3))
+    b = rng.uniform(0, 1, n ** 3)
+    M = gpu.Matrix.diffusion7(n, n, n, k)
+    M.set_jacobi()
+    r = M.pcg(b, tol=1e-15, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, iter_max=20000,
+              precision=gpu.PREC_MIXED, alpha_stop=0)
+    assert r["status"] == 1 and r["iters"] < 20000
+    assert np.all(np.isfinite(r["x"]))
+    A = O.gen_diffusion7(n, n, n, k)
+    assert np.linalg.norm(b - O.spmv_csr(A, r["x"])) <= 1e-12 * np.linalg.norm(b)
+
+
 def test_device_resident_solve_and_repeat(gpu):
     """Inputs resident in HBM; the same handle solves twice (ReDoIt)."""
     A, xt, b, jac = problem(40, 6)
