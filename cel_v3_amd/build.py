@@ -46,6 +46,31 @@ def _compile(src: str) -> str:
     return obj
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_OUT = os.path.join(FORTRAN_DIR, "_build")
+ROOT = os.path.dirname(HERE)
+
+
+def build_fortran(verbose: bool = True) -> str | None:
+    """Compiles the ISO_C_BINDING module (cel_gpu, cel_gpu_elmsolver) and the
+    element-solver example driver tests/fortran/elm_poisson.f90 against
+    libcel_gpu.so.  Returns the driver path, or None without amdflang."""
+    fc = os.environ.get("FC") or shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc) and not shutil.which(fc):
+        return None
+    os.makedirs(FORTRAN_OUT, exist_ok=True)
+    mod_obj = os.path.join(FORTRAN_OUT, "cel_gpu_mod.o")
+    subprocess.run([fc, "-O2", "-c", os.path.join(FORTRAN_DIR, "cel_gpu_mod.f90"), "-o", mod_obj,
+                    "-J", FORTRAN_OUT], check=True, cwd=FORTRAN_OUT)
+    exe = os.path.join(FORTRAN_OUT, "elm_poisson")
+    subprocess.run([fc, "-O2", os.path.join(ROOT, "tests", "fortran", "elm_poisson.f90"), mod_obj,
+                    "-I", FORTRAN_OUT, "-L", HERE, "-lcel_gpu", "-Wl,-rpath,$ORIGIN/../..", "-o", exe],
+                   check=True, cwd=FORTRAN_OUT)
+    if verbose:
+        print(f"built {exe}")
+    return exe
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -64,3 +89,4 @@ def build(verbose: bool = True) -> str:
 
 if __name__ == "__main__":
     build()
+    build_fortran()
