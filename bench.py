@@ -10,10 +10,12 @@ x_true ~ U[0,1) (numpy default_rng(1)), x0 = 0, stopped at
 ||b - A x|| <= tol * ||b||.  Inputs are resident in HBM before the timed
 region starts.  Prints one JSON line on stdout; progress goes to stderr.
 
-The FP64 solves run with the one-subdomain model of the reference's
-distributed SpMV: the 256^3 block carries six halo faces (65,536 entries
-each) and packs its six send faces every SpMV; with no neighbour present the
-received values are zero, so the solve is the Dirichlet problem.
+The timed mixed and FP64 solves use the same local block.  One more FP64
+solve runs the one-subdomain model of the reference's distributed SpMV: the
+256^3 block carries six halo faces (65,536 entries each) and packs its six
+send faces every SpMV; with no neighbour present the received values are
+zero, so it solves the same Dirichlet problem.  Per-kernel times come from
+HIP events around every K1 launch in separate (untimed) solves.
 """
 from __future__ import annotations
 
@@ -30,7 +32,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "256^3 PCG time-to-1e-10: mixed vs plain FP64 GPU; SpMV GB/s as % HBM peak"
 HBM_PEAK_GBS = 8000.0
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_pcg_spmv_fp64_256.traffic")
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "hbm_traffic_256.txt")
 
 
 def log(msg):
@@ -64,14 +66,17 @@ def face_blocks(N):
     return out
 
 
-def read_traffic():
-    """HBM bytes per K1 launch from the committed rocprofv3 PMC summary."""
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (lines "traffic_bytes_per_launch <kernel> = <bytes>"), or None."""
     try:
         with open(PROFILE_TRAFFIC) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch"):
-                    return float(line.split("=")[1])
-    except OSError:
+                if line.startswith("traffic_bytes_per_launch") and "=" in line:
+                    key, val = line.split("=", 1)
+                    if key.split(None, 1)[1].strip() == kernel:
+                        return float(val)
+    except (OSError, IndexError, ValueError):
         pass
     return None
 
@@ -124,8 +129,11 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--format", default="csr", choices=["csr", "jad"])
     ap.add_argument("--skip-cpu", action="store_true", help="do not time the CPU baseline")
-    ap.add_argument("--only", default="all", choices=["all", "fp64", "mixed", "spmv"],
-                    help="run one part only (profiling runs)")
+    ap.add_argument("--only", default="all", choices=["all", "fp64", "mixed", "spmv", "profile"],
+                    help="run one part only; 'profile' = one streaming copy of known size "
+                         "(counter calibration) and FP64, FP64-with-halo and mixed solves "
+                         "capped at --iters iterations, for hardware-counter runs")
+    ap.add_argument("--iters", type=int, default=20, help="iteration cap of --only profile")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -174,6 +182,22 @@ def main():
         return rel, err
 
     out = {}
+    if args.only == "profile":
+        copy_bytes = 256 * 1024 ** 2
+        ms = cel.stream_copy_time(copy_bytes, reps=2)
+        prof = {"copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes,
+                "copy_ms": ms}
+        for name, prec, halo in (("fp64", cel.PREC_FP64, 0), ("fp64_halo", cel.PREC_FP64, use_halo),
+                                 ("mixed", cel.PREC_MIXED, 0)):
+            x = cel.DeviceArray(M.n)
+            r = M.pcg(d_b, x0=None, tol=args.tol, tol_mode=cel.TOL_REL_B, x0_mode=cel.X0_ZERO,
+                      iter_max=args.iters, precision=prec, fmt=fmt, use_halo=halo, profile=1)
+            prof[name] = {"iters": r["iters"], "spmv_bytes": r["spmv_bytes"],
+                          "mean_spmv_ms": r["spmv_ms"] / max(r["spmv_launches"], 1)}
+            log(f"profile {name}: {r['iters']} iterations, K1 "
+                f"{prof[name]['mean_spmv_ms'] * 1e3:.1f} us")
+        print(json.dumps({"profile": prof, "n": M.n, "nnz": M.nnz}))
+        return
     # ------------------------------------------------------------ SpMV kernels alone
     if args.only in ("all", "spmv"):
         log("SpMV kernels")
@@ -187,15 +211,21 @@ def main():
             ms, nb = M.spmv_time(x, y, fmt=f, precision=prec, reps=20)
             spmv[name] = {"ms": ms, "bytes": nb, "GBs": nb / ms * 1e-6,
                           "frac_hbm": nb / ms * 1e-6 / HBM_PEAK_GBS}
-        if fmt == cel.FMT_JAD:
-            ms, nb = M.spmv_time(x64, y64, fmt=cel.FMT_JAD, precision=cel.PREC_FP64, reps=20)
-            spmv["jad_fp64"] = {"ms": ms, "bytes": nb, "GBs": nb / ms * 1e-6,
-                                "frac_hbm": nb / ms * 1e-6 / HBM_PEAK_GBS}
+        # the JAD form of the same block, converted on the device
+        if fmt != cel.FMT_JAD:
+            t0 = time.perf_counter()
+            M.to_jad()
+            spmv["jad_convert_ms"] = (time.perf_counter() - t0) * 1e3
+        for name, prec, x, y in (("jad_fp64", cel.PREC_FP64, x64, y64),
+                                 ("jad_fp32", cel.PREC_MIXED, x32, y32)):
+            ms, nb = M.spmv_time(x, y, fmt=cel.FMT_JAD, precision=prec, reps=20)
+            spmv[name] = {"ms": ms, "bytes": nb, "GBs": nb / ms * 1e-6,
+                          "frac_hbm": nb / ms * 1e-6 / HBM_PEAK_GBS}
         cb = 2 * 1024 ** 3
         ms = cel.stream_copy_time(cb, reps=10)
         spmv["stream_copy_GBs"] = 2 * cb / ms * 1e-6
         for k, v in spmv.items():
-            log(f"  {k}: {v if not isinstance(v, dict) else round(v['GBs'], 1)} GB/s")
+            log(f"  {k}: " + (f"{v['GBs']:.1f} GB/s" if isinstance(v, dict) else f"{v:.1f}"))
         out["spmv"] = spmv
         del x64, y64, x32, y32
 
@@ -207,11 +237,13 @@ def main():
     for w in range(args.warmup):
         log(f"warm-up {w + 1}/{args.warmup}")
         if args.only in ("all", "fp64"):
-            solve(cel.PREC_FP64, halo=use_halo)
+            solve(cel.PREC_FP64)
         if args.only in ("all", "mixed"):
             solve(cel.PREC_MIXED)
 
     # ------------------------------------------------------------ timed region
+    # both forms solve the same local 256^3 block without halo blocks, so
+    # the mixed/FP64 ratio compares like with like; neither is profiled
     fp64_runs, mixed_runs = [], []
     cel.lib().cel_gpu_synchronize()
     t_start = time.perf_counter()
@@ -226,19 +258,33 @@ def main():
     t_start = time.perf_counter()
     for k in range(args.steps):
         if args.only in ("all", "fp64"):
-            r = solve(cel.PREC_FP64, profile=1, halo=use_halo)
+            r = solve(cel.PREC_FP64)
             fp64_runs.append(r)
             log(f"step {k + 1}: fp64 {r['solve_ms']:.1f} ms, {r['iters']} iterations")
     cel.lib().cel_gpu_synchronize()
     t_fp64 = time.perf_counter() - t_start
+
+    # ------------------------------------------------------------ per-kernel timing (untimed)
+    # one more solve of each form with HIP events around every K1 launch on
+    # the library stream, and the one-subdomain halo model of the FP64 form
+    prof = {}
+    if args.only in ("all", "mixed"):
+        prof["mixed"] = solve(cel.PREC_MIXED, profile=1)
+    if args.only in ("all", "fp64"):
+        prof["fp64"] = solve(cel.PREC_FP64, profile=1)
+        if use_halo:
+            prof["fp64_halo"] = solve(cel.PREC_FP64, profile=1, halo=1)
+
+    def k1(r):
+        ms = r["spmv_ms"] / max(r["spmv_launches"], 1)
+        return ms, r["spmv_bytes"] / (ms * 1e-3) / 1e9
 
     res = dict(metric=METRIC, n_gpus=1, steps=args.steps, warmup=args.warmup,
                higher_is_better=False, scaling="weak", vs_baseline=None, data="synthetic",
                unit="s")
     cfg = {"workload": f"poisson7_{N}^3_jacobi_pcg_tol{args.tol:g}", "n": M.n, "nnz": M.nnz,
            "format": args.format, "x_true": "U[0,1) numpy default_rng(1)", "x0": 0,
-           "stop": "||b-Ax|| <= tol*||b||", "halo_faces_fp64": 6 if use_halo else 0,
-           "parallelism": "1 GPU, one subdomain"}
+           "stop": "||b-Ax|| <= tol*||b||", "parallelism": "1 GPU, one subdomain"}
     if mixed_runs:
         ms = [r["solve_ms"] for r in mixed_runs]
         rel, err = check(mixed_runs[-1])
@@ -248,26 +294,18 @@ def main():
         res["mixed"] = {"solve_ms": ms, "inner_iters": mixed_runs[-1]["iters"],
                         "fp64_replacements": mixed_runs[-1]["outer_iters"],
                         "true_rel_residual": rel, "max_abs_err_vs_x_true": err,
-                        "wall_ms_per_step": t_mixed / args.steps * 1e3}
+                        "wall_ms_per_step": t_mixed / args.steps * 1e3,
+                        "iter_bytes": mixed_runs[-1]["iter_bytes"]}
         log(f"mixed: {res['value']:.4f} s, true rel residual {rel:.3e}, max err {err:.3e}")
     if fp64_runs:
         ms = [r["solve_ms"] for r in fp64_runs]
         rel, err = check(fp64_runs[-1])
         last = fp64_runs[-1]
-        k1_ms = last["spmv_ms"] / max(last["spmv_launches"], 1)
-        achieved = last["spmv_bytes"] / (k1_ms * 1e-3) / 1e9
-        traffic = read_traffic()
         res["fp64"] = {"solve_ms": ms, "iters": last["iters"], "true_rel_residual": rel,
                        "max_abs_err_vs_x_true": err, "wall_ms_per_step": t_fp64 / args.steps * 1e3,
                        "iter_bytes": last["iter_bytes"],
-                       "iter_GBs": last["iter_bytes"] * last["iters"] / (ms[-1] * 1e-3) / 1e9}
-        res["roofline"] = {"kernel": "k_pcg_spmv<double,CSR,halo> (v = A p fused with v.p)",
-                           "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                           "algorithmic_bytes": last["spmv_bytes"], "mean_launch_ms": k1_ms,
-                           "launches": last["spmv_launches"]}
-        log(f"fp64: {np.median(ms) / 1e3:.4f} s, {last['iters']} iterations, K1 {k1_ms * 1e3:.1f} us "
-            f"= {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%} of 8 TB/s)")
+                       "iter_GBs": last["iter_bytes"] * last["iters"] / (np.median(ms) * 1e-3) / 1e9}
+        log(f"fp64: {np.median(ms) / 1e3:.4f} s, {last['iters']} iterations")
         if "value" not in res:
             res["value"] = float(np.median(ms)) / 1e3
             res["ms_per_step"] = t_fp64 / args.steps * 1e3
@@ -275,6 +313,29 @@ def main():
     if mixed_runs and fp64_runs:
         res["speedup_mixed_vs_fp64"] = (np.median([r["solve_ms"] for r in fp64_runs]) /
                                         np.median([r["solve_ms"] for r in mixed_runs]))
+    # the dominant kernel of the timed (mixed) solve is its FP32 K1; the FP64
+    # K1 is reported beside it
+    for key, form, name in (("roofline", "mixed", "k_pcg_spmv<float,CSR> (v = A p fused with v.p)"),
+                            ("roofline_fp64", "fp64", "k_pcg_spmv<double,CSR> (v = A p fused with v.p)"),
+                            ("roofline_fp64_halo", "fp64_halo",
+                             "k_pcg_spmv<double,CSR,halo> (+6 halo faces, one-subdomain model)")):
+        if form not in prof:
+            continue
+        r = prof[form]
+        k1_ms, achieved = k1(r)
+        res[key] = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": read_traffic(name.split(" ")[0]),
+                    "algorithmic_bytes": r["spmv_bytes"], "mean_launch_ms": k1_ms,
+                    "launches": r["spmv_launches"], "solve_ms_profiled": r["solve_ms"]}
+        log(f"{key}: {name.split(' ')[0]} {k1_ms * 1e3:.1f} us = {achieved:.0f} GB/s "
+            f"({achieved / HBM_PEAK_GBS:.1%} of 8 TB/s)")
+    if "fp64_halo" in prof:
+        r = prof["fp64_halo"]
+        res["fp64_halo"] = {"solve_ms": r["solve_ms"], "iters": r["iters"],
+                            "halo_faces": 6, "face_entries": 6 * N * N,
+                            "note": "one-subdomain model: 6 halo blocks and face packing every "
+                                    "SpMV, received values zero"}
     res["config"] = cfg
     res.update(out)
     if not args.skip_cpu and fp64_runs and args.only == "all":

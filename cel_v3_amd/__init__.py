@@ -38,7 +38,8 @@ EXPORTS = [
     "cel_gpu_mat_get_block", "cel_gpu_mat_set_send_list", "cel_gpu_face_pack",
     "cel_gpu_spmv_distr", "cel_gpu_mat_ghost_len", "cel_gpu_pcg_params_default",
     "cel_gpu_pcg_solve", "cel_gpu_cg_adjoint", "cel_gpu_diffusion7_kappa_grad",
-    "cel_gpu_mat_add_halo_block", "cel_gpu_stream_copy_time",
+    "cel_gpu_mat_add_halo_block", "cel_gpu_stream_copy_time", "cel_gpu_batch_create",
+    "cel_gpu_batch_destroy", "cel_gpu_batch_tune", "cel_gpu_batch_info", "cel_gpu_batch_solve",
 ]
 
 
@@ -387,6 +388,82 @@ def params(**kw) -> PcgParams:
             raise TypeError(f"unknown PCG parameter {k}")
         setattr(p, k, v)
     return p
+
+
+# ---------------------------------------------------------------- batches
+class Batch:
+    """Independent systems (and blocks of right-hand sides sharing one
+    matrix) solved together: cel_gpu_batch_*.  b and x of a system with k
+    right-hand sides are arrays of shape (k, n) (k = 1: shape (n,))."""
+
+    def __init__(self, mats, nrhs=None):
+        self.mats = list(mats)  # keeps the matrices alive
+        self.nrhs = [1] * len(self.mats) if nrhs is None else [int(k) for k in nrhs]
+        hs = (_vp * len(self.mats))(*[m.h for m in self.mats])
+        ks = np.asarray(self.nrhs, dtype=np.int32)
+        self.h = _vp()
+        _check(lib().cel_gpu_batch_create(C.byref(self.h), len(self.mats), hs, _ptr(ks)))
+        self.lanes = sum(self.nrhs)
+
+    def _vecs(self, arrays):
+        out = []
+        for m, k, a in zip(self.mats, self.nrhs, arrays):
+            a = _f64(a)
+            if a.size != m.n * k:
+                raise ValueError(f"expected {k} x {m.n} values, got {a.shape}")
+            out.append(a)
+        return out
+
+    def tune(self, bs, iters=24):
+        bs = self._vecs(bs)
+        ptrs = (_vp * len(bs))(*[_ptr(b) for b in bs])
+        _check(lib().cel_gpu_batch_tune(self.h, ptrs, MEM_HOST, int(iters)))
+        return self.info()
+
+    def info(self):
+        lanes, grid, chunk, weight, reb = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        el, ms = C.c_longlong(), C.c_double()
+        _check(lib().cel_gpu_batch_info(self.h, C.byref(lanes), C.byref(el), C.byref(grid),
+                                        C.byref(chunk), C.byref(weight), C.byref(reb),
+                                        C.byref(ms)))
+        return {"lanes": lanes.value, "elements": el.value, "grid": grid.value,
+                "chunk": chunk.value, "weight": weight.value, "rebalance": reb.value,
+                "ms_per_iter": ms.value}
+
+    def solve(self, bs, x0s=None, tol=1e-9, tol_mode=TOL_REL_B, iter_max=10000, x0_mode=X0_ZERO,
+              allow_not_converged=True, **kw):
+        """Returns {"x": [arrays shaped like b], "lanes": [stats dicts], "total": stats dict}."""
+        p = params(tol=tol, tol_mode=tol_mode, iter_max=iter_max, x0_mode=x0_mode, **kw)
+        p.memspace = MEM_HOST
+        bs = self._vecs(bs)
+        if x0s is not None:
+            xs = [a.copy() for a in self._vecs(x0s)]
+            p.x0_mode = X0_USER
+        else:
+            xs = [np.zeros(b.size) for b in bs]
+        bp = (_vp * len(bs))(*[_ptr(b) for b in bs])
+        xp = (_vp * len(xs))(*[_ptr(x) for x in xs])
+        lane_st = (PcgStats * self.lanes)()
+        tot = PcgStats()
+        rc = lib().cel_gpu_batch_solve(self.h, bp, xp, C.byref(p), lane_st, C.byref(tot))
+        if rc < 0 and allow_not_converged:
+            pass  # per-lane statuses say which lanes failed
+        else:
+            _check(rc, allow=(OK, NOT_CONVERGED) if allow_not_converged else (OK,))
+        shaped = [x.reshape(k, m.n) if k > 1 else x for x, k, m in zip(xs, self.nrhs, self.mats)]
+        return {"x": shaped, "lanes": [st.as_dict() for st in lane_st], "total": tot.as_dict(),
+                "rc": rc}
+
+    def close(self):
+        if self.h:
+            lib().cel_gpu_batch_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def stream_copy_time(nbytes, reps=20):

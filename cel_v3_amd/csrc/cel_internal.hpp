@@ -142,6 +142,15 @@ struct Mat {
 
 void pcg_release_workspace(Mat* A);
 
+}  // namespace cel
+
+// the opaque handle of the C ABI
+struct cel_gpu_mat {
+    cel::Mat m;
+};
+
+namespace cel {
+
 // ---------------------------------------------------------------- kernels (host launchers)
 // All launches go to rt().stream.
 int launch_gen_poisson7(int nx, int ny, int nz, const double* kappa, int* xadj, int* cols,

@@ -8,8 +8,8 @@
 //    +-nx*ny columns) are then read by the same XCD and hit its L2.
 //  * Reductions are deterministic: a fixed per-thread order, a fixed
 //    butterfly inside the wavefront, a fixed order across waves, one partial
-//    per logical block, and the last block to finish sums the partials in
-//    logical-block order.
+//    per logical block; the consuming kernel sums the partials in
+//    logical-block order (sum_partials), in every block that needs them.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -82,26 +82,6 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* smem /*
     __syncthreads();
 }
 
-// Publishes this block's partial(s) (already stored by thread 0) and
-// returns true in every thread of the block that finished last.  The
-// counter is reset by that block, so the same counter serves the next launch.
-__device__ __forceinline__ bool last_block(unsigned* counter)
-{
-    __shared__ bool is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned prev = atomicAdd(counter, 1u);
-        is_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (is_last) {
-        __threadfence();
-        if (threadIdx.x == 0) *counter = 0u;
-    }
-    return is_last;
-}
-
 // Sum of G partials in logical-block order by the whole block: thread t
 // sums t, t+256, ... sequentially, then block_sum.  Valid in thread 0.
 __device__ __forceinline__ double sum_partials(const double* partials, int G, double* smem)
@@ -109,39 +89,6 @@ __device__ __forceinline__ double sum_partials(const double* partials, int G, do
     double s = 0.0;
     for (int i = threadIdx.x; i < G; i += kBlock) s = s + __builtin_nontemporal_load(partials + i);
     return block_sum(s, smem);
-}
-
-// Reduction chunks: the rows are cut into chunks of chunk_rows (a multiple
-// of 256 that depends only on n), every chunk produces one partial sum, and
-// logical block lb owns the contiguous chunk range [c0, c1).  The sums are
-// therefore independent of the grid size and of the occupancy.
-__host__ __device__ __forceinline__ int chunk_rows_for(long long n)
-{
-    long long tiles = (n + 255) / 256;
-    long long ct = tiles / 2048;
-    if (ct < 1) ct = 1;
-    if (ct > 32) ct = 32;
-    return (int)(ct * 256);
-}
-
-__device__ __forceinline__ void chunk_range(int nchunks, int G, int lb, int& c0, int& c1)
-{
-    const int per = (nchunks + G - 1) / G;
-    c0 = min(lb * per, nchunks);
-    c1 = min(c0 + per, nchunks);
-}
-
-// contiguous element range [lo, hi) of logical block lb out of G for n items,
-// aligned to `align`
-__device__ __forceinline__ void block_range(long long n, int G, int lb, int align, long long& lo,
-                                            long long& hi)
-{
-    long long per = (n + G - 1) / G;
-    per = (per + align - 1) / align * align;
-    lo = per * lb;
-    hi = lo + per;
-    if (lo > n) lo = n;
-    if (hi > n) hi = n;
 }
 
 }  // namespace cel

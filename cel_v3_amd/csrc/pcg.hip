@@ -140,32 +140,65 @@ __device__ __forceinline__ const double* jad_vals<double>(const MatView& A) { re
 template <>
 __device__ __forceinline__ const float* jad_vals<float>(const MatView& A) { return A.jvals32; }
 
-// Row sum of one 256-row tile in the requested format.  Returns the sum of
-// row `row` (set here).  For CSR row = r0 + tid; for JAD row = perm[r0+tid].
-// HALO adds the halo entries after the local sum ((0 + local) + halo...).
+// Walks the tiles [t0, t1) of one block in the requested format and
+// returns, per tile, the calling thread's row sum and its row.  For CSR
+// row = r0 + tid (the next tile's loads are issued while this tile's
+// gathers wait); for JAD row = perm[r0 + tid].  HALO adds the halo entries
+// after the local sum ((0 + local) + halo...).  With xrow, x[row] is
+// loaded too (before the next tile's loads).  next() contains
+// __syncthreads() for CSR: every thread of the block calls it.
 template <typename VT, typename AT, int FMT, bool HALO>
-__device__ __forceinline__ AT tile_rows(const MatView& A, int t, const VT* __restrict__ x,
-                                        CsrStage<VT>& sm, int* smap, int& row)
-{
-    const int r0 = t * kTileRows;
-    AT s;
-    if constexpr (FMT == CEL_FMT_CSR) {
-        s = csr_tile<VT, AT>(A.n, r0, A.xadj, A.cols, csr_vals<VT>(A), x, sm);
-        row = r0 + threadIdx.x;
-        s = AT(0) + s;
-        if constexpr (HALO)
-            s = halo_tile<AT>(r0, t, s, A.hrow_tile, A.hrows, A.hptr, A.hcols, A.hvals, A.ghost, smap);
-    } else {
-        const int k = r0 + threadIdx.x;
-        s = AT(0);
-        row = A.n;
-        if (k < A.n) {
-            s = jad_row<VT, AT>(k, A.mmax, A.jbegin, A.jcols, jad_vals<VT>(A), x);
-            row = A.jperm[k];
+struct TileWalk {
+    CsrTileStream<VT> cs;
+
+    // Contains __syncthreads() for CSR: every thread of the block calls it.
+    __device__ __forceinline__ void start(const MatView& A, long long t0, long long t1,
+                                          CsrStage<VT>& sm)
+    {
+        if constexpr (FMT == CEL_FMT_CSR) {
+            if (t0 < t1) {  // uniform across the block
+                cs.fetch(A.n, (int)t0, true, A.xadj, A.cols, csr_vals<VT>(A));
+                cs.stage(sm);
+                __syncthreads();
+            }
         }
     }
-    return s;
-}
+
+    __device__ __forceinline__ AT next(const MatView& A, long long t, long long t1,
+                                       const VT* __restrict__ x, CsrStage<VT>& sm, int& row,
+                                       VT* xrow)
+    {
+        const int r0 = (int)t * kTileRows;
+        AT s;
+        if constexpr (FMT == CEL_FMT_CSR) {
+            const int nxt = (t + 1 < t1) ? (int)(t + 1) : -1;
+            row = r0 + threadIdx.x;
+            s = cs.template rows<AT>(A.n, (int)t, nxt, A.xadj, A.cols, csr_vals<VT>(A), x, sm, row,
+                                     xrow);
+            __syncthreads();  // every row of the tile is summed
+            if (nxt >= 0) {   // uniform across the block
+                cs.stage(sm);
+                __syncthreads();
+            }
+            s = AT(0) + s;
+            if constexpr (HALO) {
+                if (row < A.n)
+                    s = halo_row<AT>(row, (int)t, s, A.hrow_tile, A.hrows, A.hptr, A.hcols,
+                                     A.hvals, A.ghost);
+            }
+        } else {
+            const int k = r0 + threadIdx.x;
+            s = AT(0);
+            row = A.n;
+            if (k < A.n) {
+                s = jad_row<VT, AT>(k, A.mmax, A.jbegin, A.jcols, jad_vals<VT>(A), x);
+                row = A.jperm[k];
+                if (xrow) *xrow = x[row];
+            }
+        }
+        return s;
+    }
+};
 
 // ------------------------------------------------------------------ residual kernels
 // r = b - A x with the FP64 operator; per-block partials: r.z, r.r, b.b.
@@ -192,7 +225,6 @@ __global__ __launch_bounds__(kBlock) void k_residual(MatView A, const double* __
         }
     }
     __shared__ CsrStage<double> sm;
-    __shared__ int smap[kBlock];
     __shared__ double red[3 * kWaves];
     const int lb = logical_block();
     const int G = gridDim.x;
@@ -201,9 +233,11 @@ __global__ __launch_bounds__(kBlock) void k_residual(MatView A, const double* __
     unit_range(ntiles, G, lb, t0, t1);
     const bool restart = REPLACE && P.ru_restart;
     double arz = 0.0, arr = 0.0, abb = 0.0;
+    TileWalk<double, double, FMT, HALO> walk;
+    walk.start(A, t0, t1, sm);
     for (long long t = t0; t < t1; ++t) {
         int row;
-        double s = tile_rows<double, double, FMT, HALO>(A, (int)t, x, sm, smap, row);
+        double s = walk.next(A, t, t1, x, sm, row, nullptr);
         if (row < A.n) {
             const double bi = b[row];
             const double ri = bi - s;  // r = 0; r = r + A x; r = b - r
@@ -244,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(MatView A, const double* __
 // replacement partials into the token first (every block, same order) and
 // decides whether the iteration runs.
 template <typename VT, int FMT, bool HALO>
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv(MatView A, const VT* __restrict__ p,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pcg_spmv(MatView A, const VT* __restrict__ p,
                                                      VT* __restrict__ v, double* __restrict__ part,
                                                      const double* __restrict__ rpart, int Gr,
                                                      const Ctrl* __restrict__ cin,
@@ -287,19 +321,64 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(MatView A, const VT* __rest
         return;
     }
     __shared__ CsrStage<VT> sm;
-    __shared__ int smap[kBlock];
     const int lb = logical_block();
     const long long ntiles = (A.n + kTileRows - 1) / kTileRows;
     long long t0, t1;
     unit_range(ntiles, gridDim.x, lb, t0, t1);
     double acc = 0.0;
-    for (long long t = t0; t < t1; ++t) {
-        int row;
-        VT s = tile_rows<VT, VT, FMT, HALO>(A, (int)t, p, sm, smap, row);
-        if (row < A.n) {
-            v[row] = s;
-            double prod = double(s) * double(p[row]);
-            acc = acc + prod;
+    if constexpr (FMT == CEL_FMT_CSR) {
+        // the halo products of each tile are computed into LDS while the
+        // tile is staged and added after the local sum, in block order
+        __shared__ HaloStage hs;
+        CsrTileStream<VT> cs;
+        const VT* vals = csr_vals<VT>(A);
+        bool hstaged = true;
+        if constexpr (HALO) {
+            halo_stage_init(hs);
+            __syncthreads();
+        }
+        if (t0 < t1) {  // uniform across the block
+            cs.fetch(A.n, (int)t0, true, A.xadj, A.cols, vals);
+            cs.stage(sm);
+            if constexpr (HALO)
+                hstaged = halo_stage_mark(hs, (int)t0, A.hrow_tile, A.hrows, A.hptr, A.hcols,
+                                          A.hvals, A.ghost);
+            __syncthreads();
+        }
+        for (long long t = t0; t < t1; ++t) {
+            const int nxt = (t + 1 < t1) ? (int)(t + 1) : -1;
+            const int row = (int)t * kTileRows + threadIdx.x;
+            VT pr = VT(0);
+            VT s = cs.template rows<VT>(A.n, (int)t, nxt, A.xadj, A.cols, vals, p, sm, row, &pr);
+            s = VT(0) + s;
+            if constexpr (HALO)
+                s = halo_stage_add<VT>(hs, (int)t, hstaged, s, A.hcols, A.hvals, A.ghost);
+            __syncthreads();  // the tile's LDS stage and halo products are consumed
+            if (nxt >= 0) {
+                cs.stage(sm);
+                if constexpr (HALO)
+                    hstaged = halo_stage_mark(hs, nxt, A.hrow_tile, A.hrows, A.hptr, A.hcols,
+                                              A.hvals, A.ghost);
+                __syncthreads();
+            }
+            if (row < A.n) {
+                v[row] = s;
+                double prod = double(s) * double(pr);
+                acc = acc + prod;
+            }
+        }
+    } else {
+        TileWalk<VT, VT, FMT, false> walk;
+        walk.start(A, t0, t1, sm);
+        for (long long t = t0; t < t1; ++t) {
+            int row;
+            VT pr = VT(0);
+            VT s = walk.next(A, t, t1, p, sm, row, &pr);
+            if (row < A.n) {
+                v[row] = s;
+                double prod = double(s) * double(pr);
+                acc = acc + prod;
+            }
         }
     }
     const double bs = block_sum(acc, red);

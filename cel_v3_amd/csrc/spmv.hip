@@ -18,10 +18,21 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmv(int n, const int* __restric
     __shared__ CsrStage<VT> sm;
     const int lb = logical_block();
     const int t0 = lb * tpb, t1 = min(t0 + tpb, ntiles);
+    CsrTileStream<VT> ts;
+    if (t0 < t1) {  // uniform across the block
+        ts.fetch(n, t0, true, xadj, cols, vals);
+        ts.stage(sm);
+        __syncthreads();
+    }
     for (int t = t0; t < t1; ++t) {
-        const int r0 = t * kTileRows;
-        VT s = csr_tile<VT, VT>(n, r0, xadj, cols, vals, x, sm);
-        const int row = r0 + threadIdx.x;
+        const int row = t * kTileRows + threadIdx.x;
+        const int nxt = t + 1 < t1 ? t + 1 : -1;
+        VT s = ts.template rows<VT>(n, t, nxt, xadj, cols, vals, x, sm, row, nullptr);
+        __syncthreads();
+        if (nxt >= 0) {
+            ts.stage(sm);
+            __syncthreads();
+        }
         if (row < n) y[row] = accumulate ? y[row] + s : VT(0) + s;
     }
 }

@@ -2,12 +2,19 @@
 // and the fused PCG kernels.
 //
 // CSR tile (CSR-stream): one 256-row tile per block iteration.  The tile's
-// entries [xadj[r0], xadj[r0+256]) are contiguous, so the block first
-// streams values and column indices into LDS with coalesced loads, then
-// every thread sums its own row from LDS in column order.  That keeps the
-// reference's per-row summation order (src/fop/cel_fop_mv_module.f90:215-229)
-// while the HBM stream stays fully coalesced.  Tiles whose entries do not
-// fit the LDS stage fall back to direct per-row loads (same order).
+// entries [xadj[r0], xadj[r0+256]) are contiguous, so the block streams
+// values and column indices with coalesced loads into registers (8 entries
+// per thread, all loads issued before the first wait), parks them in LDS,
+// and every thread then sums *its own row* from LDS in column order.  That
+// keeps the reference's per-row summation order
+// (src/fop/cel_fop_mv_module.f90:215-229) while the HBM stream stays fully
+// coalesced.  The loads of tile t+1 are issued before the row sums of tile
+// t, so they are in flight while tile t's gathers of x wait.  Tiles whose
+// entries do not fit the LDS stage fall back to direct per-row loads (same
+// order).
+//
+// Row sums gather up to 8 x values at once before adding them in column
+// order, so a 7-point row costs one round trip to L2, not seven.
 //
 // JAD tile: thread k of the tile owns permuted row k; diagonal d stores
 // entry k at begin[d] + k, so consecutive threads read consecutive
@@ -20,99 +27,326 @@
 namespace cel {
 
 constexpr int kTileRows = kBlock;
-constexpr int kStageCap = 2048;  // staged entries per tile (8 per row on average)
+constexpr int kStageCap = 2048;            // staged entries per tile (8 per row on average)
+constexpr int kPer = kStageCap / kBlock;   // staged entries per thread
+constexpr int kGather = 8;                 // x values gathered per round trip
 
 template <typename VT>
 struct CsrStage {
     VT val[kStageCap];
     int col[kStageCap];
-    int ptr[kTileRows + 1];
 };
 
-// Returns the row sum of row r0 + threadIdx.x (0 for threads past n).
-// Contains __syncthreads(); every thread of the block must call it.
-template <typename VT, typename AT>
-__device__ __forceinline__ AT csr_tile(int n, int r0, const int* __restrict__ xadj,
-                                       const int* __restrict__ cols,
-                                       const VT* __restrict__ vals, const VT* __restrict__ x,
-                                       CsrStage<VT>& sm)
+// sum + sum_{start <= j < len} a[j] * x[c[j]] in j order, gathering
+// kGather x values per round trip.  LDS or global a/c (NT: non-temporal
+// global loads).
+template <typename AT, typename VT, bool NT>
+__device__ __forceinline__ AT row_sum_from(AT sum, const VT* a, const int* c, int start, int len,
+                                           const VT* __restrict__ x)
 {
-    const int tid = threadIdx.x;
-    const int nrows = min(kTileRows, n - r0);
-    // nrows + 1 row pointers: a full tile needs 257 of them, one more than
-    // the block has threads
-    for (int k = tid; k <= nrows; k += kBlock) sm.ptr[k] = xadj[r0 + k];
-    __syncthreads();
-    const int e0 = sm.ptr[0];
-    const int cnt = sm.ptr[nrows] - e0;
-    AT sum = AT(0);
-    if (cnt <= kStageCap) {
-        for (int k = tid; k < cnt; k += kBlock) {
-            sm.val[k] = ld_stream(vals + e0 + k);
-            sm.col[k] = ld_stream(cols + e0 + k);
+    for (int b = start; b < len; b += kGather) {
+        VT av[kGather], xv[kGather];
+#pragma unroll
+        for (int i = 0; i < kGather; ++i) {
+            if (b + i < len) {
+                const int col = NT ? ld_stream(c + b + i) : c[b + i];
+                av[i] = NT ? ld_stream(a + b + i) : a[b + i];
+                xv[i] = x[col];
+            }
         }
-        __syncthreads();
-        if (tid < nrows) {
-            const int j1 = sm.ptr[tid + 1] - e0;
-            for (int j = sm.ptr[tid] - e0; j < j1; ++j) {
-                AT prod = AT(sm.val[j]) * AT(x[sm.col[j]]);
+#pragma unroll
+        for (int i = 0; i < kGather; ++i) {
+            if (b + i < len) {
+                AT prod = AT(av[i]) * AT(xv[i]);
                 sum = sum + prod;
             }
         }
-    } else if (tid < nrows) {
-        const int j1 = sm.ptr[tid + 1];
-        for (int j = sm.ptr[tid]; j < j1; ++j) {
-            AT prod = AT(ld_stream(vals + j)) * AT(x[ld_stream(cols + j)]);
-            sum = sum + prod;
-        }
     }
-    __syncthreads();
     return sum;
 }
 
-// JAD: permuted row k = k0 + threadIdx.x.  dlen[d] = entries of diagonal d
-// (non-increasing), begin[d] = start of diagonal d (0-based).
+// Buffer resource over `bytes` bytes at `base` (gfx9 raw buffer, stride 0):
+// loads at offsets >= bytes return 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const void* base, int bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes,
+                                             0x00020000);
+}
+
+// Non-temporal buffer load (aux bit 1 = nt on gfx950).
+template <typename T>
+__device__ __forceinline__ T buf_load_nt(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ double buf_load_nt<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));
+}
+template <>
+__device__ __forceinline__ float buf_load_nt<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2));
+}
+template <>
+__device__ __forceinline__ int buf_load_nt<int>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2);
+}
+
+// Software-pipelined CSR tile stream.  fetch(t) issues the loads of tile t
+// (row bounds of every thread and, when the tile fits the stage, its
+// entries into registers); stage() parks them in LDS; rows() sums the
+// calling thread's row of the staged tile while the next tile loads:
+//   fetch(t0); stage; sync
+//   for t: s = rows(t, next = t+1); sync; stage; sync
+template <typename VT>
+struct CsrTileStream {
+    VT v[kPer];
+    int c[kPer];
+    int e0, cnt;  // first entry and entries of the fetched tile (uniform)
+    int lo, hi;   // the calling thread's row bounds (lo = hi = e0 past n)
+
+    // Loads of tile t.  Unconditional by design: with want = false (no next
+    // tile; t is then the current tile) or a tile larger than the stage,
+    // the buffer covers 0 bytes and the loads return 0 without touching
+    // memory, so the registers never merge loaded and stale values (such
+    // merges force a wait for the loads before the current tile's work).
+    __device__ __forceinline__ void fetch(int n, int t, bool want, const int* __restrict__ xadj,
+                                          const int* __restrict__ cols,
+                                          const VT* __restrict__ vals)
+    {
+        const int tid = threadIdx.x;
+        const int r0 = t * kTileRows;
+        const int nrows = min(kTileRows, n - r0);
+        e0 = xadj[r0];
+        cnt = xadj[r0 + nrows] - e0;
+        // threads past the last row get lo = hi (an empty row)
+        lo = xadj[r0 + min(tid, nrows)];
+        hi = xadj[r0 + min(tid + 1, nrows)];
+        const int m = (want && cnt <= kStageCap) ? cnt : 0;
+        const auto rv = tile_rsrc(vals + e0, m * (int)sizeof(VT));
+        const auto rc = tile_rsrc(cols + e0, m * (int)sizeof(int));
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            v[i] = buf_load_nt<VT>(rv, tid * (int)sizeof(VT), i * kBlock * (int)sizeof(VT));
+            c[i] = buf_load_nt<int>(rc, tid * (int)sizeof(int), i * kBlock * (int)sizeof(int));
+        }
+    }
+
+    // Parks the fetched entries in LDS (tiles that fit the stage).
+    __device__ __forceinline__ void stage(CsrStage<VT>& sm)
+    {
+        if (cnt <= kStageCap) {  // uniform across the block
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int k = threadIdx.x + i * kBlock;
+                if (k < cnt) {
+                    sm.val[k] = v[i];
+                    sm.col[k] = c[i];
+                }
+            }
+        }
+    }
+
+    // Row sum of the staged tile t; starts fetching tile `next` (< 0: none).
+    // The gathers of x (and x[row] into *xrow when xrow is not null) are
+    // issued *before* the loads of the next tile: vector memory loads
+    // complete in issue order, so a gather issued after them would wait
+    // for the whole next tile.  The fetched registers are consumed by the
+    // caller's stage() in the same iteration, never carried across it.
+    template <typename AT>
+    __device__ __forceinline__ AT rows(int n, int t, int next, const int* __restrict__ xadj,
+                                       const int* __restrict__ cols, const VT* __restrict__ vals,
+                                       const VT* __restrict__ x, const CsrStage<VT>& sm, int row,
+                                       VT* xrow)
+    {
+        return rows<AT>(n, t, next, xadj, cols, vals, x, sm, row, xrow, [] {});
+    }
+
+    // As above; pre() runs after this tile's gathers are issued and before
+    // the next tile's loads (for loads the caller needs for this tile).
+    template <typename AT, typename Pre>
+    __device__ __forceinline__ AT rows(int n, int t, int next, const int* __restrict__ xadj,
+                                       const int* __restrict__ cols, const VT* __restrict__ vals,
+                                       const VT* __restrict__ x, const CsrStage<VT>& sm, int row,
+                                       VT* xrow, Pre&& pre)
+    {
+        const int b0 = e0, l = lo - e0, len = hi - lo;
+        const bool staged = cnt <= kStageCap;  // uniform across the block
+        // branch-free gathers: entries past the row read a clamped LDS slot
+        // and x[0]; their products are not added
+        VT xv[kGather];
+        if (staged) {
+#pragma unroll
+            for (int i = 0; i < kGather; ++i) {
+                const int ci = sm.col[min(l + i, kStageCap - 1)];
+                xv[i] = x[i < len ? ci : 0];
+            }
+        }
+        VT xr = VT(0);
+        if (xrow) xr = x[row < n ? row : 0];
+        pre();
+        fetch(n, next >= 0 ? next : t, next >= 0, xadj, cols, vals);
+        if (xrow) *xrow = xr;
+        AT sum = AT(0);
+        if (staged) {
+#pragma unroll
+            for (int i = 0; i < kGather; ++i) {
+                const VT a = sm.val[min(l + i, kStageCap - 1)];
+                AT prod = AT(a) * AT(xv[i]);
+                AT tt = sum + prod;
+                sum = i < len ? tt : sum;
+            }
+            if (len > kGather)  // rows longer than kGather continue in column order
+                sum = row_sum_from<AT, VT, false>(sum, sm.val + l, sm.col + l, kGather, len, x);
+        } else {
+            sum = row_sum_from<AT, VT, true>(sum, vals + b0 + l, cols + b0 + l, 0, len, x);
+        }
+        return sum;
+    }
+};
+
+// JAD: permuted row k = k0 + threadIdx.x.  begin[d+1] - begin[d] = entries
+// of diagonal d (non-increasing), begin[d] = start of diagonal d (0-based).
+// kGather diagonals per round trip; the sum runs in diagonal order.
 template <typename VT, typename AT>
 __device__ __forceinline__ AT jad_row(int k, int mmax, const int* __restrict__ begin,
                                       const int* __restrict__ jcols,
                                       const VT* __restrict__ jvals, const VT* __restrict__ x)
 {
     AT sum = AT(0);
-    for (int d = 0; d < mmax; ++d) {
-        const int b = begin[d];
-        if (k >= begin[d + 1] - b) break;
-        AT prod = AT(ld_stream(jvals + b + k)) * AT(x[ld_stream(jcols + b + k)]);
+    for (int d0 = 0; d0 < mmax; d0 += kGather) {
+        VT av[kGather], xv[kGather];
+        bool on[kGather];
+#pragma unroll
+        for (int i = 0; i < kGather; ++i) {
+            const int d = d0 + i;
+            on[i] = false;
+            if (d < mmax) {
+                const int b = begin[d];
+                on[i] = k < begin[d + 1] - b;
+                if (on[i]) {
+                    const int col = ld_stream(jcols + b + k);
+                    av[i] = ld_stream(jvals + b + k);
+                    xv[i] = x[col];
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kGather; ++i) {
+            if (on[i]) {
+                AT prod = AT(av[i]) * AT(xv[i]);
+                sum = sum + prod;
+            }
+        }
+        if (!on[kGather - 1]) break;
+    }
+    return sum;
+}
+
+// Halo entries (rows of this rank with columns owned by a neighbour).
+// hrow_tile[t]..hrow_tile[t+1] index the compact, ascending list of rows of
+// tile t that have halo entries; entries of compact row q are
+// hptr[q]..hptr[q+1] in block order, and their columns are ghost slots.
+//
+// HaloStage holds, per block, the halo part of the tile being summed:
+// while tile t is staged (a phase that waits for the tile's loads anyway)
+// the block computes every halo product hvals[j] * ghost[hcols[j]] of the
+// tile into LDS and marks, per tile row, its tag (= t), first product and
+// count.  After the staging barrier each thread adds its row's products
+// from LDS in block order -- no global loads in the row epilogue.  Tags
+// make clearing unnecessary (a stale entry carries an older tile number).
+// A tile with more than kHaloStage halo entries reads them from global
+// memory instead (same order, same products).
+constexpr int kHaloStage = 1024;
+
+struct HaloStage {
+    double prod[kHaloStage];
+    int tag[kTileRows];
+    int first[kTileRows];  // LDS slot, or global entry index for an unstaged tile
+    int count[kTileRows];
+};
+
+// All threads; call once before the first mark, followed by a barrier.
+__device__ __forceinline__ void halo_stage_init(HaloStage& hs) { hs.tag[threadIdx.x] = -1; }
+
+// Stages tile t's halo products and row marks (before the barrier that
+// precedes the tile's rows).  Returns whether the products are in LDS
+// (uniform across the block).
+__device__ __forceinline__ bool halo_stage_mark(HaloStage& hs, int t,
+                                                const int* __restrict__ hrow_tile,
+                                                const int* __restrict__ hrows,
+                                                const int* __restrict__ hptr,
+                                                const int* __restrict__ hcols,
+                                                const double* __restrict__ hvals,
+                                                const double* __restrict__ ghost)
+{
+    const int r0 = t * kTileRows;
+    const int q0 = hrow_tile[t], q1 = hrow_tile[t + 1];
+    if (q0 == q1) return true;
+    const int e0 = hptr[q0], e1 = hptr[q1];
+    const bool staged = e1 - e0 <= kHaloStage;
+    if (staged) {
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlock) {
+            double prod = hvals[e] * ghost[hcols[e]];
+            hs.prod[e - e0] = prod;
+        }
+    }
+    for (int q = q0 + (int)threadIdx.x; q < q1; q += kBlock) {
+        const int i = hrows[q] - r0;
+        const int j = hptr[q];
+        hs.tag[i] = t;
+        hs.first[i] = staged ? j - e0 : j;
+        hs.count[i] = hptr[q + 1] - j;
+    }
+    return staged;
+}
+
+// sum + the calling thread's halo products of tile t, in block order.
+template <typename AT>
+__device__ __forceinline__ AT halo_stage_add(const HaloStage& hs, int t, bool staged, AT sum,
+                                             const int* __restrict__ hcols,
+                                             const double* __restrict__ hvals,
+                                             const double* __restrict__ ghost)
+{
+    if (hs.tag[threadIdx.x] != t) return sum;
+    const int f = hs.first[threadIdx.x], c = hs.count[threadIdx.x];
+    for (int k = 0; k < c; ++k) {
+        AT prod;
+        if (staged) {
+            prod = AT(hs.prod[f + k]);
+        } else {
+            prod = AT(hvals[f + k]) * AT(ghost[hcols[f + k]]);
+        }
         sum = sum + prod;
     }
     return sum;
 }
 
-// Halo entries of one tile.  hrow_tile[t]..hrow_tile[t+1] index the compact
-// list of rows with halo entries (ascending); entries of compact row q are
-// hptr[q]..hptr[q+1] in block order, columns are ghost slots.  Adds them to
-// `sum` of the calling thread's row in that order.  Contains __syncthreads().
+// Halo entries of row `row` of tile t found by binary search in the tile's
+// compact rows (no LDS map; used where a kernel runs once per solve).
 template <typename AT>
-__device__ __forceinline__ AT halo_tile(int r0, int t, AT sum, const int* __restrict__ hrow_tile,
-                                        const int* __restrict__ hrows,
-                                        const int* __restrict__ hptr,
-                                        const int* __restrict__ hcols,
-                                        const double* __restrict__ hvals,
-                                        const double* __restrict__ ghost, int* smap)
+__device__ __forceinline__ AT halo_row(int row, int t, AT sum, const int* __restrict__ hrow_tile,
+                                       const int* __restrict__ hrows,
+                                       const int* __restrict__ hptr,
+                                       const int* __restrict__ hcols,
+                                       const double* __restrict__ hvals,
+                                       const double* __restrict__ ghost)
 {
-    const int q0 = hrow_tile[t], q1 = hrow_tile[t + 1];
-    if (q0 == q1) return sum;  // uniform across the block
-    smap[threadIdx.x] = -1;
-    __syncthreads();
-    for (int q = q0 + threadIdx.x; q < q1; q += kBlock) smap[hrows[q] - r0] = q;
-    __syncthreads();
-    const int q = smap[threadIdx.x];
-    if (q >= 0) {
-        for (int j = hptr[q]; j < hptr[q + 1]; ++j) {
+    int lo = hrow_tile[t];
+    const int end = hrow_tile[t + 1];
+    if (lo == end) return sum;
+    int hi = end;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (hrows[mid] < row) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < end && hrows[lo] == row) {
+        for (int j = hptr[lo]; j < hptr[lo + 1]; ++j) {
             AT prod = AT(hvals[j]) * AT(ghost[hcols[j]]);
             sum = sum + prod;
         }
     }
-    __syncthreads();
     return sum;
 }
 

@@ -273,6 +273,35 @@ int cel_gpu_cg_adjoint(cel_gpu_mat* A, const double* x, const double* dLdx, doub
 int cel_gpu_diffusion7_kappa_grad(int nx, int ny, int nz, const double* kappa, const double* x,
                                   const double* lambda, double* grad, int memspace);
 
+/* ---- batched Jacobi-PCG (no reference counterpart) --------------------- */
+/* Many independent SPD systems -- uneven sizes, and blocks of right-hand
+ * sides that share one matrix -- solved together by one ring of kernels
+ * replayed as a hipGraph.  Each right-hand side is a lane with its own
+ * scalars and the reference's op sequence (cel_cgalg_jacobi,
+ * src/fcgalg/cel_cgalg_module.f90:178-305, called once per system there).
+ * FP64, CSR, no halo blocks.  The matrices (set_jacobi done) must outlive
+ * the batch.  nrhs[s] in 1..64 (NULL: all 1). */
+typedef struct cel_gpu_batch cel_gpu_batch;
+int cel_gpu_batch_create(cel_gpu_batch** B, int nsys, cel_gpu_mat* const* mats, const int* nrhs);
+int cel_gpu_batch_destroy(cel_gpu_batch* B);
+/* Times candidate launch configurations (grid size, iterations per graph
+ * replay, block apportionment) on `iters` iterations of the batch with the
+ * right-hand sides b and keeps the fastest for later solves.  The choice
+ * changes how dot products are grouped, so results agree with an untuned
+ * solve to rounding, not bit for bit. */
+int cel_gpu_batch_tune(cel_gpu_batch* B, const double* const* b, int memspace, int iters);
+/* The configuration in use and the time per iteration the tuner measured
+ * (0 before tuning).  weight 0: blocks dealt by entries x lanes, 1: by rows x lanes. */
+int cel_gpu_batch_info(const cel_gpu_batch* B, int* lanes, long long* elements, int* grid,
+                       int* chunk, int* weight, int* rebalance, double* ms_per_iter);
+/* b[s], x[s]: n_s x nrhs[s] column-major (right-hand side j at offset j*n_s),
+ * host or device per params->memspace.  lane_stats (may be NULL) receives one
+ * entry per right-hand side in system order; total: status (most severe),
+ * iters (maximum), outer_iters (graph replays), solve_ms, iter_bytes. */
+int cel_gpu_batch_solve(cel_gpu_batch* B, const double* const* b, double* const* x,
+                        const cel_pcg_params* params, cel_pcg_stats* lane_stats,
+                        cel_pcg_stats* total);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,3 +72,41 @@ def spmv_matrices():
         for name in ("s7_6x6x6", "s27_5x5x5", "s19_4x5x6"):
             out.append(("ref_" + name, gold_csr(name)[0]))
     return out
+
+
+def face_coeffs(shape, kappa):
+    """Interior face coefficients 2 k_i k_j / (k_i + k_j) of the diffusion7
+    matrix along x, y and z (arrays indexed [z, y, x])."""
+    nx, ny, nz = shape
+    k3 = kappa.reshape(nz, ny, nx)
+    out = []
+    for ax, m in ((2, nx), (1, ny), (0, nz)):
+        a = np.take(k3, range(0, m - 1), axis=ax)
+        c = np.take(k3, range(1, m), axis=ax)
+        out.append((2.0 * a * c) / (a + c))
+    return out
+
+
+def delta_apply(shape, kappa, kappa_p, f0, v):
+    """(A(kappa_p) - A(kappa)) v from the changes of the face coefficients:
+    row i gets sum_f da_f (v_i - v_j) over its interior faces and
+    dkappa_i v_i per boundary face.  Nothing here is rounded to the size of
+    a matrix entry, only to the size of the change."""
+    nx, ny, nz = shape
+    v3 = v.reshape(nz, ny, nx)
+    out = np.zeros_like(v3)
+    for ax, m, a0, a1 in zip((2, 1, 0), (nx, ny, nz), f0, face_coeffs(shape, kappa_p)):
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[ax] = slice(0, m - 1)
+        hi[ax] = slice(1, m)
+        t = (a1 - a0) * (v3[tuple(lo)] - v3[tuple(hi)])
+        out[tuple(lo)] += t
+        out[tuple(hi)] -= t
+    dk = (kappa_p - kappa).reshape(nz, ny, nx)
+    for ax, m in ((2, nx), (1, ny), (0, nz)):
+        for end in (0, m - 1):
+            sl = [slice(None)] * 3
+            sl[ax] = end
+            out[tuple(sl)] += dk[tuple(sl)] * v3[tuple(sl)]
+    return out.ravel()

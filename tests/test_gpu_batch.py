@@ -1,0 +1,204 @@
+"""Batched Jacobi-PCG (cel_gpu_batch_*): systems of uneven size and blocks of
+right-hand sides sharing one matrix, solved by one ring of kernels.
+
+Every right-hand side is a lane that runs the reference's op sequence
+(src/fcgalg/cel_cgalg_module.f90:206-304) with its own scalars; only the
+grouping of the dot-product sums differs from a single-system solve.  Each
+lane is therefore checked against the oracle's FP64 PCG (iterations +-1,
+solution to 1e-10 of its size at equal iteration counts) and against the
+single-system GPU solver.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+S = 1e8  # right-hand sides scaled to this norm: the absolute |alpha| test cannot fire first
+
+
+def small_batch(gpu, seed=11):
+    rng = np.random.default_rng(seed)
+    shapes = [(6, 7, 5), (9, 9, 9), (4, 5, 6), (12, 10, 8), (3, 3, 3)]
+    mats, bs, nrhs, hosts = [], [], [], []
+    for sh in shapes:
+        n = int(np.prod(sh))
+        kappa = np.exp(rng.normal(0.0, 0.5, n))
+        M = gpu.Matrix.diffusion7(*sh, kappa)
+        M.set_jacobi()
+        b = rng.random(n)
+        mats.append(M)
+        bs.append(b * (S / np.linalg.norm(b)))
+        nrhs.append(1)
+        hosts.append(O.gen_diffusion7(*sh, kappa))
+    for sh, k in (((8, 8, 8), 3), ((10, 9, 8), 16)):
+        n = int(np.prod(sh))
+        M = gpu.Matrix.poisson7(*sh)
+        M.set_jacobi()
+        B = rng.uniform(-1.0, 1.0, (k, n))
+        B *= S / np.linalg.norm(B, axis=1, keepdims=True)
+        mats.append(M)
+        bs.append(B)
+        nrhs.append(k)
+        hosts.append(O.gen_poisson7(*sh))
+    return mats, bs, nrhs, hosts
+
+
+def lanes_of(bs, nrhs):
+    out = []
+    for b, k in zip(bs, nrhs):
+        out.extend([b] if k == 1 else list(b))
+    return out
+
+
+def test_batch_matches_oracle_lane_by_lane(gpu):
+    mats, bs, nrhs, hosts = small_batch(gpu)
+    batch = gpu.Batch(mats, nrhs)
+    tol = 1e-9
+    out = batch.solve(bs, tol=tol, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, alpha_stop=1)
+    assert out["rc"] == 0, out["total"]
+    xs = lanes_of(out["x"], nrhs)
+    lane = 0
+    for s, (A, b, k) in enumerate(zip(hosts, bs, nrhs)):
+        jac = O.set_jacobi(O.csr_diagonal(A))
+        for j in range(k):
+            bj = b if k == 1 else b[j]
+            ref = O.pcg_jacobi(A, jac, bj, x0_mode=O.X0_ZERO, eps=tol * np.linalg.norm(bj),
+                               iter_max=10000, workers=3)
+            st = out["lanes"][lane]
+            assert st["status"] == 0
+            assert abs(st["iters"] - ref["iters"]) <= 1, (s, j, st["iters"], ref["iters"])
+            if st["iters"] == ref["iters"]:
+                np.testing.assert_allclose(xs[lane], ref["x"], rtol=0,
+                                           atol=1e-10 * np.abs(ref["x"]).max())
+                assert abs(st["final_norm"] - ref["norm"]) <= 1e-8 * ref["norm"]
+            lane += 1
+    assert lane == batch.lanes == 5 + 3 + 16
+
+
+def test_batch_matches_single_system_solver(gpu):
+    mats, bs, nrhs, _ = small_batch(gpu, seed=12)
+    batch = gpu.Batch(mats, nrhs)
+    out = batch.solve(bs, tol=1e-9, tol_mode=gpu.TOL_REL_B, alpha_stop=0)
+    xs = lanes_of(out["x"], nrhs)
+    lane = 0
+    for M, b, k in zip(mats, bs, nrhs):
+        for j in range(k):
+            bj = b if k == 1 else b[j]
+            single = M.pcg(bj, tol=1e-9, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, alpha_stop=0)
+            st = out["lanes"][lane]
+            assert abs(st["iters"] - single["iters"]) <= 1
+            if st["iters"] == single["iters"]:
+                np.testing.assert_allclose(xs[lane], single["x"], rtol=0,
+                                           atol=1e-10 * np.abs(single["x"]).max())
+            lane += 1
+
+
+def test_batch_repeat_is_bitwise_and_tuning_keeps_results(gpu):
+    mats, bs, nrhs, _ = small_batch(gpu, seed=13)
+    batch = gpu.Batch(mats, nrhs)
+    a = batch.solve(bs, tol=1e-10, alpha_stop=0)
+    b = batch.solve(bs, tol=1e-10, alpha_stop=0)
+    for xa, xb in zip(a["x"], b["x"]):
+        np.testing.assert_array_equal(xa, xb)
+    info = batch.tune(bs, iters=16)
+    assert info["grid"] >= len(mats) and info["chunk"] in (8, 32) and info["ms_per_iter"] > 0
+    c = batch.solve(bs, tol=1e-10, alpha_stop=0)
+    for la, lc in zip(a["lanes"], c["lanes"]):
+        assert lc["status"] == 0 and abs(la["iters"] - lc["iters"]) <= 1
+    for xa, xc in zip(a["x"], c["x"]):
+        np.testing.assert_allclose(xa, xc, rtol=0, atol=1e-8 * np.abs(xa).max())
+
+
+def test_batch_iteration_cap_and_nonfinite_lane(gpu):
+    mats, bs, nrhs, _ = small_batch(gpu, seed=14)
+    batch = gpu.Batch(mats, nrhs)
+    out = batch.solve(bs, tol=1e-12, iter_max=5, alpha_stop=0)
+    assert out["total"]["status"] == 1
+    assert all(st["iters"] <= 5 for st in out["lanes"])
+    assert all(st["status"] == 1 for st in out["lanes"] if st["iters"] == 5)
+    bad = [b.copy() for b in bs]
+    bad[1][3] = np.nan
+    out = batch.solve(bad, tol=1e-9, alpha_stop=0)
+    assert out["rc"] == -7
+    sts = [st["status"] for st in out["lanes"]]
+    assert sts[1] == -7
+    assert all(s == 0 for i, s in enumerate(sts) if i != 1)
+
+
+def test_batch_rejects_bad_arguments(gpu):
+    M = gpu.Matrix.poisson7(4, 4, 4)
+    M.set_jacobi()
+    with pytest.raises(gpu.CelError):
+        gpu.Batch([M], [0])
+    with pytest.raises(gpu.CelError):
+        gpu.Batch([M], [65])
+    N = gpu.Matrix.poisson7(4, 4, 4)  # no Jacobi vector
+    with pytest.raises(gpu.CelError):
+        gpu.Batch([M, N])
+
+
+def config4(gpu):
+    """64 variable-coefficient 7-point systems, edge ~ U{24..96}, kappa =
+    exp(g), g ~ N(0, 0.5^2), b ~ U[0,1), plus the 128^3 Poisson system with
+    16 right-hand sides ~ U[-1,1].  One numpy default_rng(3) stream, drawn in
+    this order: the 64 edges, then per system its kappa and its b, then the
+    16 x 128^3 right-hand sides."""
+    rng = np.random.default_rng(3)
+    edges = rng.integers(24, 97, size=64)
+    mats, bs, nrhs = [], [], []
+    for e in edges:
+        n = int(e) ** 3
+        kappa = np.exp(rng.normal(0.0, 0.5, n))
+        b = rng.random(n)
+        M = gpu.Matrix.diffusion7(int(e), int(e), int(e), kappa)
+        M.set_jacobi()
+        mats.append(M)
+        bs.append(b)
+        nrhs.append(1)
+    P = gpu.Matrix.poisson7(128, 128, 128)
+    P.set_jacobi()
+    mats.append(P)
+    bs.append(rng.uniform(-1.0, 1.0, (16, 128 ** 3)))
+    nrhs.append(16)
+    return edges, mats, bs, nrhs
+
+
+def test_batch_config4_workload(gpu):
+    import time
+
+    edges, mats, bs, nrhs = config4(gpu)
+    batch = gpu.Batch(mats, nrhs)
+    info = batch.tune(bs, iters=32)
+    print(f"tuned: {info}", flush=True)
+    t0 = time.perf_counter()
+    out = batch.solve(bs, tol=1e-9, tol_mode=gpu.TOL_REL_B, alpha_stop=0, iter_max=20000)
+    wall = time.perf_counter() - t0
+    tot = out["total"]
+    print(f"batch: {tot['solve_ms']:.1f} ms device, {wall:.2f} s wall, max {tot['iters']} iterations, "
+          f"{tot['outer_iters']} graph replays, lanes {batch.lanes}", flush=True)
+    assert out["rc"] == 0
+    assert all(st["status"] == 0 for st in out["lanes"])
+    xs = lanes_of(out["x"], nrhs)
+    lanes_b = lanes_of(bs, nrhs)
+    owners = [m for m, k in zip(mats, nrhs) for _ in range(k)]
+    worst = 0.0
+    for M, b, x in zip(owners, lanes_b, xs):
+        rel = np.linalg.norm(b - M.spmv(x)) / np.linalg.norm(b)
+        worst = max(worst, rel)
+    print(f"worst true relative residual {worst:.3e}", flush=True)
+    assert worst <= 2e-9
+    # spot checks against the single-system solver; the sum of their times
+    # is the sequential alternative
+    seq_ms = 0.0
+    for i in (int(np.argmin(edges)), int(np.argmax(edges)), 64):
+        M, k = mats[i], nrhs[i]
+        for j in ([0] if k == 1 else [0, 15]):
+            b = bs[i] if k == 1 else bs[i][j]
+            lane = sum(nrhs[:i]) + j
+            single = M.pcg(b, tol=1e-9, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, alpha_stop=0)
+            seq_ms += single["solve_ms"]
+            assert abs(out["lanes"][lane]["iters"] - single["iters"]) <= 1, (i, j)
+            np.testing.assert_allclose(xs[lane], single["x"], rtol=0,
+                                       atol=1e-6 * np.abs(single["x"]).max())

@@ -480,3 +480,25 @@ def test_sddmm_brute_force():
     full = -np.outer(lam, x)
     rows = np.repeat(np.arange(20), np.diff(A.xadj))
     np.testing.assert_array_equal(g, full[rows, A.cols - 1])
+
+
+def test_face_difference_operator_matches_generated_matrices():
+    """delta_apply (the finite-difference perturbation of the adjoint test)
+    equals (A(kappa') - A(kappa)) v built from two generated matrices, to the
+    rounding of the change."""
+    import scipy.sparse as sp
+
+    from helpers import delta_apply, face_coeffs
+
+    for shape in ((5, 4, 3), (6, 6, 6)):
+        n = int(np.prod(shape))
+        rng = np.random.default_rng(7)
+        kappa = np.exp(rng.normal(0.0, 1.0, n))
+        kp = kappa * np.exp(1e-3 * rng.normal(size=n))
+        A0 = O.gen_diffusion7(*shape, kappa)
+        A1 = O.gen_diffusion7(*shape, kp)
+        dA = sp.csr_matrix((A1.vals - A0.vals, A0.cols - 1, A0.xadj - 1), shape=(n, n))
+        v = rng.normal(size=n)
+        ref = dA @ v
+        got = delta_apply(shape, kappa, kp, face_coeffs(shape, kappa), v)
+        assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
