@@ -225,3 +225,38 @@ def test_profile_counts_spmv_launches(gpu):
     b = np.random.default_rng(0).random(M.n)
     r = M.pcg(b, tol=1e-8, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, profile=1)
     assert r["spmv_launches"] >= r["iters"] and r["spmv_ms"] > 0
+
+
+def test_config3_mixed_384(gpu):
+    """The precision workload: 7-point Poisson on 384^3 (n = 56,623,104,
+    nnz = 395,476,992), x_true ~ N(0,1) (numpy default_rng(2)), b = A x_true
+    (FP64 device SpMV), x0 = 0, FP32 inner iteration with FP64 residual
+    replacement to 1e-10 relative residual.  Checked at full size by what
+    does not need a CPU solve: the FP64 true residual, the error against the
+    plain FP64 GPU solve of the same system (the FP64 path is pinned against
+    the oracle at the smaller sizes), and the error against x_true."""
+    n = 384
+    M = gpu.Matrix.poisson7(n, n, n)
+    M.set_jacobi()
+    assert M.nnz == 395476992
+    xt = np.random.default_rng(2).normal(size=M.n)
+    d_xt = gpu.DeviceArray.from_host(xt)
+    d_b = gpu.DeviceArray(M.n)
+    M.spmv(d_xt, d_b)
+    b = d_b.to_host()
+    kw = dict(tol=1e-10, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, iter_max=20000)
+    mixed = M.pcg(d_b, precision=gpu.PREC_MIXED, allow_not_converged=False, **kw)
+    fp64 = M.pcg(d_b, precision=gpu.PREC_FP64, allow_not_converged=False, **kw)
+    xm, xf = mixed["x"].to_host(), fp64["x"].to_host()
+    res = gpu.DeviceArray(M.n)
+    M.spmv(mixed["x"], res)
+    true_rel = np.linalg.norm(b - res.to_host()) / np.linalg.norm(b)
+    d_fp64 = np.abs(xm - xf).max() / np.abs(xf).max()
+    err = np.abs(xm - xt).max()
+    print(f"384^3 mixed {mixed['solve_ms']:.1f} ms ({mixed['iters']} inner, {mixed['outer_iters']} "
+          f"replacements), fp64 {fp64['solve_ms']:.1f} ms ({fp64['iters']} iterations); true rel "
+          f"{true_rel:.3e}, max|x_mixed - x_fp64|/max|x| {d_fp64:.3e}, max err vs x_true {err:.3e}",
+          flush=True)
+    assert true_rel <= 1e-10
+    assert d_fp64 <= 1e-7
+    assert abs(mixed["iters"] - fp64["iters"]) <= 0.05 * fp64["iters"]

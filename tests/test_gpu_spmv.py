@@ -170,3 +170,28 @@ def test_coo_create_matches_to_csr(gpu):
     xa, ca, va = M.get_csr()
     np.testing.assert_array_equal(xa, O.coo_to_csr(d.n, rows, 0))
     np.testing.assert_array_equal(ca, d.cols)
+
+
+def test_jad_256_conversion_and_spmv(gpu):
+    """The 256^3 block of the bench converted on the device: jad_perm is the
+    stable descending sort of the row lengths (ties in ascending row order),
+    diagonal d holds entry d of every row in perm order, and the JAD SpMV
+    equals the CSR SpMV bit for bit (same per-row order)."""
+    n = 256
+    M = gpu.Matrix.poisson7(n, n, n)
+    M.to_jad()
+    j = M.get_jad()
+    xadj, cols, vals = M.get_csr()
+    lens = np.diff(xadj)
+    perm0 = j["perm"] - 1
+    np.testing.assert_array_equal(perm0, np.argsort(-lens, kind="stable"))
+    begin0 = j["begin"] - 1
+    for d in range(j["mmax"]):
+        k = begin0[d + 1] - begin0[d]
+        rows = perm0[:k]
+        assert (lens[rows] > d).all() and (k == M.n or lens[perm0[k]] <= d)
+        src = xadj[rows] - 1 + d
+        np.testing.assert_array_equal(j["cols"][begin0[d]:begin0[d + 1]], cols[src])
+        np.testing.assert_array_equal(j["vals"][begin0[d]:begin0[d + 1]], vals[src])
+    x = np.random.default_rng(5).normal(size=M.n)
+    np.testing.assert_array_equal(M.spmv(x, fmt=gpu.FMT_JAD), M.spmv(x))
