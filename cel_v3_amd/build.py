@@ -53,8 +53,9 @@ ROOT = os.path.dirname(HERE)
 
 def build_fortran(verbose: bool = True) -> str | None:
     """Compiles the ISO_C_BINDING module (cel_gpu, cel_gpu_elmsolver) and the
-    element-solver example driver tests/fortran/elm_poisson.f90 against
-    libcel_gpu.so.  Returns the driver path, or None without amdflang."""
+    Fortran drivers tests/fortran/elm_poisson.f90 (element-solver API) and
+    tests/fortran/batch_poisson.f90 (batched solves) against libcel_gpu.so.
+    Returns the first driver's path, or None without amdflang."""
     fc = os.environ.get("FC") or shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     if not os.path.exists(fc) and not shutil.which(fc):
         return None
@@ -62,12 +63,15 @@ def build_fortran(verbose: bool = True) -> str | None:
     mod_obj = os.path.join(FORTRAN_OUT, "cel_gpu_mod.o")
     subprocess.run([fc, "-O2", "-c", os.path.join(FORTRAN_DIR, "cel_gpu_mod.f90"), "-o", mod_obj,
                     "-J", FORTRAN_OUT], check=True, cwd=FORTRAN_OUT)
-    exe = os.path.join(FORTRAN_OUT, "elm_poisson")
-    subprocess.run([fc, "-O2", os.path.join(ROOT, "tests", "fortran", "elm_poisson.f90"), mod_obj,
-                    "-I", FORTRAN_OUT, "-L", HERE, "-lcel_gpu", "-Wl,-rpath,$ORIGIN/../..", "-o", exe],
-                   check=True, cwd=FORTRAN_OUT)
-    if verbose:
-        print(f"built {exe}")
+    exe = None
+    for name in ("elm_poisson", "batch_poisson"):
+        exe_n = os.path.join(FORTRAN_OUT, name)
+        subprocess.run([fc, "-O2", os.path.join(ROOT, "tests", "fortran", name + ".f90"), mod_obj,
+                        "-I", FORTRAN_OUT, "-L", HERE, "-lcel_gpu", "-Wl,-rpath,$ORIGIN/../..",
+                        "-o", exe_n], check=True, cwd=FORTRAN_OUT)
+        if verbose:
+            print(f"built {exe_n}")
+        exe = exe if exe else exe_n
     return exe
 
 

@@ -115,6 +115,60 @@ module cel_gpu
       type(cel_pcg_stats), intent(out) :: stats
       type(c_ptr), value :: hist
     end function
+    integer(c_int) function cel_gpu_mat_gen_diffusion7(A, nx, ny, nz, kappa, memspace) &
+        bind(c, name="cel_gpu_mat_gen_diffusion7")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), intent(out) :: A
+      integer(c_int), value :: nx, ny, nz, memspace
+      real(c_double), intent(in) :: kappa(*)
+    end function
+    integer(c_int) function cel_gpu_cg_adjoint(A, x, dLdx, dLdb, dLda, params, stats) &
+        bind(c, name="cel_gpu_cg_adjoint")
+      import :: c_int, c_ptr, c_double, cel_pcg_params, cel_pcg_stats
+      type(c_ptr), value :: A
+      real(c_double), intent(in) :: x(*), dLdx(*)
+      real(c_double), intent(out) :: dLdb(*)
+      type(c_ptr), value :: dLda
+      type(cel_pcg_params), intent(in) :: params
+      type(cel_pcg_stats), intent(out) :: stats
+    end function
+    integer(c_int) function cel_gpu_diffusion7_kappa_grad(nx, ny, nz, kappa, x, lambda, grad, &
+        memspace) bind(c, name="cel_gpu_diffusion7_kappa_grad")
+      import :: c_int, c_double
+      integer(c_int), value :: nx, ny, nz, memspace
+      real(c_double), intent(in) :: kappa(*), x(*), lambda(*)
+      real(c_double), intent(out) :: grad(*)
+    end function
+    ! batched solves: mats, b and x are arrays of C pointers (c_loc of each
+    ! system's matrix handle / n x nrhs column-major array)
+    integer(c_int) function cel_gpu_batch_create(B, nsys, mats, nrhs) &
+        bind(c, name="cel_gpu_batch_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(out) :: B
+      integer(c_int), value :: nsys
+      type(c_ptr), intent(in) :: mats(*)
+      integer(c_int), intent(in) :: nrhs(*)
+    end function
+    integer(c_int) function cel_gpu_batch_destroy(B) bind(c, name="cel_gpu_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: B
+    end function
+    integer(c_int) function cel_gpu_batch_tune(B, bptrs, memspace, iters) &
+        bind(c, name="cel_gpu_batch_tune")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: B
+      type(c_ptr), intent(in) :: bptrs(*)
+      integer(c_int), value :: memspace, iters
+    end function
+    integer(c_int) function cel_gpu_batch_solve(B, bptrs, xptrs, params, lane_stats, total) &
+        bind(c, name="cel_gpu_batch_solve")
+      import :: c_int, c_ptr, cel_pcg_params, cel_pcg_stats
+      type(c_ptr), value :: B
+      type(c_ptr), intent(in) :: bptrs(*), xptrs(*)
+      type(cel_pcg_params), intent(in) :: params
+      type(cel_pcg_stats), intent(out) :: lane_stats(*)
+      type(cel_pcg_stats), intent(out) :: total
+    end function
   end interface
 
 contains
