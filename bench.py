@@ -116,6 +116,61 @@ def cpu_baseline(N, tol):
     }, r["x"]
 
 
+def batch_bench(cel, steps):
+    """64 variable-coefficient 7-point systems (edge ~ U{24..96}, kappa =
+    exp(g), g ~ N(0, 0.5^2), b ~ U[0,1)) plus the 128^3 Poisson system with
+    16 right-hand sides ~ U[-1,1], FP64 to 1e-9 relative: one tuned batch
+    against the same 80 right-hand sides solved one by one (device-resident
+    inputs, single-system solver).  Same generator stream as the batch test."""
+    rng = np.random.default_rng(3)
+    edges = rng.integers(24, 97, size=64)
+    mats, bs, nrhs = [], [], []
+    for e in edges:
+        n = int(e) ** 3
+        kappa = np.exp(rng.normal(0.0, 0.5, n))
+        b = rng.random(n)
+        M = cel.Matrix.diffusion7(int(e), int(e), int(e), kappa)
+        M.set_jacobi()
+        mats.append(M)
+        bs.append(b)
+        nrhs.append(1)
+    P = cel.Matrix.poisson7(128, 128, 128)
+    P.set_jacobi()
+    mats.append(P)
+    bs.append(rng.uniform(-1.0, 1.0, (16, 128 ** 3)))
+    nrhs.append(16)
+    batch = cel.Batch(mats, nrhs)
+    t0 = time.perf_counter()
+    info = batch.tune(bs, iters=32)
+    tune_s = time.perf_counter() - t0
+    log(f"batch tuned in {tune_s:.1f} s: {info}")
+    kw = dict(tol=1e-9, tol_mode=cel.TOL_REL_B, alpha_stop=0, iter_max=20000)
+    times = []
+    for k in range(max(1, steps)):
+        out = batch.solve(bs, **kw)
+        times.append(out["total"]["solve_ms"])
+        log(f"batch solve {k + 1}: {out['total']['solve_ms']:.1f} ms, max {out['total']['iters']} iterations")
+    lanes = [(m, b if k == 1 else b[j]) for m, b, k in zip(mats, bs, nrhs) for j in range(k)]
+    seq_ms, seq_iters = 0.0, 0
+    dev = []
+    for m, b in lanes:
+        dev.append((m, cel.DeviceArray.from_host(b)))
+    cel.lib().cel_gpu_synchronize()
+    t0 = time.perf_counter()
+    for m, d_b in dev:
+        r = m.pcg(d_b, x0_mode=cel.X0_ZERO, **kw)
+        seq_ms += r["solve_ms"]
+        seq_iters += r["iters"]
+    cel.lib().cel_gpu_synchronize()
+    seq_wall = time.perf_counter() - t0
+    log(f"one by one: {seq_ms:.1f} ms device, {seq_wall:.2f} s wall, {seq_iters} iterations in total")
+    return {"workload": "64 diffusion7 systems (edge 24..96) + poisson7 128^3 x 16 rhs, tol 1e-9",
+            "lanes": batch.lanes, "elements": info["elements"], "tuned": info,
+            "batch_ms": times, "batch_ms_median": float(np.median(times)),
+            "one_by_one_ms": seq_ms, "one_by_one_wall_s": seq_wall,
+            "speedup": seq_ms / float(np.median(times)), "tune_s": tune_s}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,10 +180,13 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--format", default="csr", choices=["csr", "jad"])
     ap.add_argument("--skip-cpu", action="store_true", help="do not time the CPU baseline")
-    ap.add_argument("--only", default="all", choices=["all", "fp64", "mixed", "spmv", "profile"],
+    ap.add_argument("--only", default="all",
+                    choices=["all", "fp64", "mixed", "spmv", "profile", "batch"],
                     help="run one part only; 'profile' = one streaming copy of known size "
                          "(counter calibration) and FP64, FP64-with-halo and mixed solves "
-                         "capped at --iters iterations, for hardware-counter runs")
+                         "capped at --iters iterations, for hardware-counter runs; 'batch' = "
+                         "the batched workload (64 variable-coefficient systems + 128^3 with 16 "
+                         "right-hand sides) against one-by-one solves")
     ap.add_argument("--iters", type=int, default=20, help="iteration cap of --only profile")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -138,6 +196,11 @@ def main():
         sys.exit(2)
 
     import cel_v3_amd as cel
+
+    if args.only == "batch":
+        cel.init(0)
+        print(json.dumps(batch_bench(cel, args.steps)))
+        return
 
     N = args.size
     cel.init(0)

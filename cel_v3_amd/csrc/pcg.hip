@@ -278,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(MatView A, const double* __
 // replacement partials into the token first (every block, same order) and
 // decides whether the iteration runs.
 template <typename VT, int FMT, bool HALO>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pcg_spmv(MatView A, const VT* __restrict__ p,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(VT) == 4 ? 5 : 4, 8))) void k_pcg_spmv(MatView A, const VT* __restrict__ p,
                                                      VT* __restrict__ v, double* __restrict__ part,
                                                      const double* __restrict__ rpart, int Gr,
                                                      const Ctrl* __restrict__ cin,
@@ -507,6 +507,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xp(long long n, XT* __res
                                                           VT* __restrict__ p,
                                                           const VT* __restrict__ r,
                                                           const VT* __restrict__ jac,
+                                                          double* __restrict__ x64,
                                                           const double* __restrict__ rpart, int G2,
                                                           const Ctrl* __restrict__ cin,
                                                           Ctrl* __restrict__ cout, Prm P)
@@ -521,45 +522,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xp(long long n, XT* __res
     const double rz = all_sum(rpart, G2, red, &bcast);
     const double rr = all_sum(rpart + G2, G2, red, &bcast);
     const double beta_d = rz / c.alpha;
-    using V = typename Vec<VT>::T;
-    constexpr int W = Vec<VT>::W;
-    const int lb = logical_block();
-    const long long unit = (long long)kBlock * W;
-    long long u0, u1;
-    unit_range((n + unit - 1) / unit, gridDim.x, lb, u0, u1);
-    const long long lo = u0 * unit, hi = min(u1 * unit, n);
     const VT sigma = VT(c.sigma), beta = VT(beta_d);
-    for (long long i = lo + (long long)threadIdx.x * W; i < hi; i += unit) {
-        if (i + W <= n) {
-            V pv = *reinterpret_cast<const V*>(p + i);
-            const V rv = *reinterpret_cast<const V*>(r + i);
-            const V jv = *reinterpret_cast<const V*>(jac + i);
-            XT xs[W];
-#pragma unroll
-            for (int k = 0; k < W; ++k) xs[k] = x[i + k];
-#pragma unroll
-            for (int k = 0; k < W; ++k) {
-                const VT pi = lane<VT>(pv, k);
-                XT t = XT(sigma * pi);
-                xs[k] = xs[k] + t;
-                VT zi = lane<VT>(rv, k) * lane<VT>(jv, k);
-                VT bp = beta * pi;
-                set_lane<VT>(pv, k, bp + zi);
-            }
-#pragma unroll
-            for (int k = 0; k < W; ++k) x[i + k] = xs[k];
-            *reinterpret_cast<V*>(p + i) = pv;
-        } else {
-            for (long long j = i; j < hi; ++j) {
-                const VT pi = p[j];
-                XT t = XT(sigma * pi);
-                x[j] = x[j] + t;
-                VT zi = r[j] * jac[j];
-                VT bp = beta * pi;
-                p[j] = bp + zi;
-            }
-        }
-    }
+    // scalars of the next iteration (they do not depend on the vectors)
     c.beta = beta_d;
     c.alpha = rz;
     c.norm2 = sqrt(rr);
@@ -580,6 +544,62 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xp(long long n, XT* __res
                 !alpha_ok(P, c.alpha) || !isfinite(c.norm2))
                    ? 1
                    : 0;
+    }
+    // mixed, replacement iteration: the FP32 correction is folded into the
+    // FP64 solution here (x64 = x64 + e, e = 0) instead of in a kernel of
+    // its own; same arithmetic
+    const bool flush = MIXED && c.ru;
+    using V = typename Vec<VT>::T;
+    constexpr int W = Vec<VT>::W;
+    const int lb = logical_block();
+    const long long unit = (long long)kBlock * W;
+    long long u0, u1;
+    unit_range((n + unit - 1) / unit, gridDim.x, lb, u0, u1);
+    const long long lo = u0 * unit, hi = min(u1 * unit, n);
+    for (long long i = lo + (long long)threadIdx.x * W; i < hi; i += unit) {
+        if (i + W <= n) {
+            V pv = *reinterpret_cast<const V*>(p + i);
+            const V rv = *reinterpret_cast<const V*>(r + i);
+            const V jv = *reinterpret_cast<const V*>(jac + i);
+            XT xs[W];
+#pragma unroll
+            for (int k = 0; k < W; ++k) xs[k] = x[i + k];
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                const VT pi = lane<VT>(pv, k);
+                XT t = XT(sigma * pi);
+                xs[k] = xs[k] + t;
+                VT zi = lane<VT>(rv, k) * lane<VT>(jv, k);
+                VT bp = beta * pi;
+                set_lane<VT>(pv, k, bp + zi);
+            }
+            if (flush) {
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    double t = (double)xs[k];
+                    x64[i + k] = x64[i + k] + t;
+                    xs[k] = XT(0);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < W; ++k) x[i + k] = xs[k];
+            *reinterpret_cast<V*>(p + i) = pv;
+        } else {
+            for (long long j = i; j < hi; ++j) {
+                const VT pi = p[j];
+                XT t = XT(sigma * pi);
+                XT xj = x[j] + t;
+                if (flush) {
+                    double tt = (double)xj;
+                    x64[j] = x64[j] + tt;
+                    xj = XT(0);
+                }
+                x[j] = xj;
+                VT zi = r[j] * jac[j];
+                VT bp = beta * pi;
+                p[j] = bp + zi;
+            }
+        }
     }
     write_ctrl(cout, c);
 }
@@ -636,6 +656,12 @@ static int build_hrow_tile(const Mat& A, Workspace& W, int ntiles)
 // Resident blocks of several kernels at once: the smallest, so every
 // variant that feeds the same consumer uses the same grid (and therefore
 // the same partials and the same sums).
+// Blocks per CU of the vector kernels (K2, K3).  Measured on the 256^3
+// solve: larger grids of these kernels, and K1 grids larger than the
+// resident blocks, were slower -- every consumer block sums all producer
+// partials, so more blocks cost more reduction work.
+constexpr int kVecBlocksPerCu = 4;
+
 static int common_grid(std::initializer_list<const void*> kernels, long long work)
 {
     int g = 1 << 30;
@@ -689,7 +715,8 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     }
     {
         const long long units = ((long long)n + 2 * kBlock - 1) / (2 * kBlock);
-        Gv = (int)std::min<long long>(rt().num_cus * 4, std::max<long long>(1, units));
+        Gv = (int)std::min<long long>((long long)rt().num_cus * kVecBlocksPerCu,
+                                      std::max<long long>(1, units));
         if (Gv >= kXcds) Gv = Gv / kXcds * kXcds;
     }
 
@@ -756,12 +783,11 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     c0.run = 1;
     CEL_HIP(hipMemcpyAsync(W.ctrl.p, &c0, sizeof(Ctrl), hipMemcpyHostToDevice, s));
     // token slots: 0 -> K1 -> 1 -> K2 -> 2 -> K3 -> 0 (FP64);
-    // mixed: K3 -> 3 -> flush -> 4 -> replacement residual -> 0
+    // mixed: K3 -> 3 -> replacement residual -> 0 (K3 folds e into x64)
     Ctrl* C0 = W.ctrl.p;
     Ctrl* C1 = W.ctrl.p + 1;
     Ctrl* C2 = W.ctrl.p + 2;
     Ctrl* C3 = W.ctrl.p + 3;
-    Ctrl* C4 = W.ctrl.p + 4;
 
     MatView V{};
     V.n = n;
@@ -858,7 +884,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
             hipLaunchKernelGGL((k_pcg_update_r<double>), dim3(Gv), dim3(kBlock), 0, s, n_ll, W.r.p, W.v.p,
                                A.jac.p, W.part2.p, W.part1.p, G1, C1, C2);
             hipLaunchKernelGGL((k_pcg_update_xp<double, double, false>), dim3(Gv), dim3(kBlock), 0, s, n_ll,
-                               dx, W.p.p, W.r.p, A.jac.p, W.part2.p, Gv, C2, C0, P);
+                               dx, W.p.p, W.r.p, A.jac.p, nullptr, W.part2.p, Gv, C2, C0, P);
         } else {
             if (fmt == CEL_FMT_CSR)
                 hipLaunchKernelGGL((k_pcg_spmv<float, CEL_FMT_CSR, false>), dim3(G1), dim3(kBlock), 0, s, V,
@@ -870,15 +896,14 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
             hipLaunchKernelGGL((k_pcg_update_r<float>), dim3(Gv), dim3(kBlock), 0, s, n_ll, W.r32.p, W.v32.p,
                                A.jac32.p, W.part2.p, W.part1.p, G1, C1, C2);
             hipLaunchKernelGGL((k_pcg_update_xp<float, float, true>), dim3(Gv), dim3(kBlock), 0, s, n_ll,
-                               W.e32.p, W.p32.p, W.r32.p, A.jac32.p, W.part2.p, Gv, C2, C3, P);
-            hipLaunchKernelGGL(k_ru_flush, dim3(Gv), dim3(kBlock), 0, s, n_ll, dx, W.e32.p, C3, C4, 0);
+                               W.e32.p, W.p32.p, W.r32.p, A.jac32.p, dx, W.part2.p, Gv, C2, C3, P);
             if (fmt == CEL_FMT_CSR)
                 hipLaunchKernelGGL((k_residual<float, CEL_FMT_CSR, false, true>), dim3(Gr), dim3(kBlock), 0, s,
-                                   V, dx, db, A.jac.p, A.jac32.p, W.r32.p, W.p32.p, W.e32.p, W.rpart.p, C4,
+                                   V, dx, db, A.jac.p, A.jac32.p, W.r32.p, W.p32.p, W.e32.p, W.rpart.p, C3,
                                    C0, P);
             else
                 hipLaunchKernelGGL((k_residual<float, CEL_FMT_JAD, false, true>), dim3(Gr), dim3(kBlock), 0, s,
-                                   V, dx, db, A.jac.p, A.jac32.p, W.r32.p, W.p32.p, W.e32.p, W.rpart.p, C4,
+                                   V, dx, db, A.jac.p, A.jac32.p, W.r32.p, W.p32.p, W.e32.p, W.rpart.p, C3,
                                    C0, P);
         }
         CEL_HIP(hipGetLastError());

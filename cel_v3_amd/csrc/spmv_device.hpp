@@ -37,30 +37,21 @@ struct CsrStage {
     int col[kStageCap];
 };
 
-// sum + sum_{start <= j < len} a[j] * x[c[j]] in j order, gathering
-// kGather x values per round trip.  LDS or global a/c (NT: non-temporal
+// sum + sum_{start <= j < len} a[j] * x[c[j]] in j order, one entry at a
+// time: the path of rows longer than kGather and of tiles larger than the
+// stage.  Kept lean on purpose -- its registers would count against the
+// occupancy of the common path.  LDS or global a/c (NT: non-temporal
 // global loads).
 template <typename AT, typename VT, bool NT>
 __device__ __forceinline__ AT row_sum_from(AT sum, const VT* a, const int* c, int start, int len,
                                            const VT* __restrict__ x)
 {
-    for (int b = start; b < len; b += kGather) {
-        VT av[kGather], xv[kGather];
-#pragma unroll
-        for (int i = 0; i < kGather; ++i) {
-            if (b + i < len) {
-                const int col = NT ? ld_stream(c + b + i) : c[b + i];
-                av[i] = NT ? ld_stream(a + b + i) : a[b + i];
-                xv[i] = x[col];
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < kGather; ++i) {
-            if (b + i < len) {
-                AT prod = AT(av[i]) * AT(xv[i]);
-                sum = sum + prod;
-            }
-        }
+#pragma unroll 1
+    for (int j = start; j < len; ++j) {
+        const int col = NT ? ld_stream(c + j) : c[j];
+        const VT av = NT ? ld_stream(a + j) : a[j];
+        AT prod = AT(av) * AT(x[col]);
+        sum = sum + prod;
     }
     return sum;
 }
