@@ -36,10 +36,12 @@
 
 #include "cel_internal.hpp"
 #include "device_common.hpp"
+#include "spmv_device.hpp"
 
 namespace cel {
 
 constexpr int kMaxRhs = 64;
+constexpr int kUnroll = 4;  // elements per thread and step in the vector kernels
 constexpr int kBStage = 2048;  // staged entries per tile
 
 struct BCtrl {
@@ -244,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
                                                   const BCtrl* __restrict__ cin,
                                                   BCtrl* __restrict__ cout, BPrm P)
 {
-    __shared__ BStage sm;
+    __shared__ CsrStage<double> sm;
     __shared__ double red[4 * 64];
     __shared__ double s1[kMaxRhs], s2[kMaxRhs], s3[kMaxRhs];
     __shared__ BCtrl lc[kMaxRhs];
@@ -289,16 +291,60 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
     const int j = threadIdx.x & (S.kp - 1);
     const bool on = lc[j].run != 0;
     const double* ps = p + S.voff;
+    double* vs = v + S.voff;
     double acc = 0.0;
-    for (int t = blk.t0; t < blk.t1; ++t) {
-        int row;
-        const double s = btile(S, t * (kBlock >> S.ksh), ps, sm, on, row);
-        if (row < S.n && on) {
-            const size_t e = S.voff + ((size_t)row << S.ksh) + j;
-            const double vi = 0.0 + s;
-            v[e] = vi;
-            double prod = vi * p[e];
-            acc = acc + prod;
+    if (S.kp == 1) {
+        // one right-hand side: the pipelined LDS tile of the single-system K1
+        // (256 rows per tile, the next tile's loads in flight meanwhile)
+        CsrTileStream<double> cs;
+        const int t0 = blk.t0, t1 = blk.t1;
+        if (t0 < t1) {
+            cs.fetch(S.n, t0, true, S.xadj, S.cols, S.vals);
+            cs.stage(sm);
+            __syncthreads();
+        }
+        for (int t = t0; t < t1; ++t) {
+            const int nxt = t + 1 < t1 ? t + 1 : -1;
+            const int row = t * kTileRows + threadIdx.x;
+            double pr = 0.0;
+            double sum = cs.template rows<double>(S.n, t, nxt, S.xadj, S.cols, S.vals, ps, sm, row,
+                                                  &pr);
+            __syncthreads();
+            if (nxt >= 0) {
+                cs.stage(sm);
+                __syncthreads();
+            }
+            if (row < S.n) {
+                const double vi = 0.0 + sum;
+                vs[row] = vi;
+                double prod = vi * pr;
+                acc = acc + prod;
+            }
+        }
+    } else {
+        // kp right-hand sides: the kp lanes of a row read the row's entries
+        // (one broadcast per wave) and gather x[col][j] contiguously; no LDS,
+        // no barriers
+        const int R = kBlock >> S.ksh;
+        const int my = threadIdx.x >> S.ksh;
+        if (on) {
+            for (int t = blk.t0; t < blk.t1; ++t) {
+                const int row = t * R + my;
+                if (row < S.n) {
+                    const int lo = S.xadj[row], hi = S.xadj[row + 1];
+                    double sum = 0.0;
+#pragma unroll 4
+                    for (int k = lo; k < hi; ++k) {
+                        double prod = S.vals[k] * ps[((size_t)S.cols[k] << S.ksh) + j];
+                        sum = sum + prod;
+                    }
+                    const size_t e = ((size_t)row << S.ksh) + j;
+                    const double vi = 0.0 + sum;
+                    vs[e] = vi;
+                    double prod = vi * ps[e];
+                    acc = acc + prod;
+                }
+            }
         }
     }
     const double bs = lane_block_sum(acc, S.kp, red);
@@ -357,7 +403,31 @@ __global__ __launch_bounds__(kBlock) void k_bupdate_r(const BSys* __restrict__ s
     double arz = 0.0, arr = 0.0;
     if (on) {
         const long long lo = blk.u0 * kBlock, hi = min(blk.u1 * (long long)kBlock, ne);
-        for (long long e = lo + threadIdx.x; e < hi; e += kBlock) {
+        // kUnroll elements per step (e, e+256, ...): their loads are in
+        // flight together; the sums still run in ascending e
+        long long e = lo + threadIdx.x;
+        for (; e + (kUnroll - 1) * kBlock < hi; e += kUnroll * kBlock) {
+            double vv[kUnroll], rv[kUnroll], jv[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const long long eu = e + u * kBlock;
+                vv[u] = v[S.voff + eu];
+                rv[u] = r[S.voff + eu];
+                jv[u] = S.jac[eu >> S.ksh];
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                double t = ms * vv[u];
+                const double ri = rv[u] + t;
+                r[S.voff + e + u * kBlock] = ri;
+                const double zi = ri * jv[u];
+                double rz = ri * zi;
+                arz = arz + rz;
+                double rr = ri * ri;
+                arr = arr + rr;
+            }
+        }
+        for (; e < hi; e += kBlock) {
             const size_t g = S.voff + e;
             double t = ms * v[g];
             const double ri = r[g] + t;
@@ -419,7 +489,28 @@ __global__ __launch_bounds__(kBlock) void k_bupdate_xp(const BSys* __restrict__ 
     const long long ne = (long long)S.n << S.ksh;
     if (on) {
         const long long lo = blk.u0 * kBlock, hi = min(blk.u1 * (long long)kBlock, ne);
-        for (long long e = lo + threadIdx.x; e < hi; e += kBlock) {
+        long long e = lo + threadIdx.x;
+        for (; e + (kUnroll - 1) * kBlock < hi; e += kUnroll * kBlock) {
+            double pv[kUnroll], xv[kUnroll], rv[kUnroll], jv[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const long long eu = e + u * kBlock;
+                pv[u] = p[S.voff + eu];
+                xv[u] = x[S.voff + eu];
+                rv[u] = r[S.voff + eu];
+                jv[u] = S.jac[eu >> S.ksh];
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const size_t g = S.voff + e + u * kBlock;
+                double t = sigma * pv[u];
+                x[g] = xv[u] + t;
+                const double zi = rv[u] * jv[u];
+                double bp = beta * pv[u];
+                p[g] = bp + zi;
+            }
+        }
+        for (; e < hi; e += kBlock) {
             const size_t g = S.voff + e;
             const double pi = p[g];
             double t = sigma * pi;
