@@ -446,9 +446,11 @@ class Batch:
         lane_st = (PcgStats * self.lanes)()
         tot = PcgStats()
         rc = lib().cel_gpu_batch_solve(self.h, bp, xp, C.byref(p), lane_st, C.byref(tot))
-        if rc < 0 and allow_not_converged:
-            pass  # per-lane statuses say which lanes failed
-        else:
+        # a lane that broke down or had a non-finite right-hand side is
+        # reported per lane (the other lanes' results are valid); any other
+        # error is raised
+        lane_errors = (-6, -7)
+        if not (allow_not_converged and rc in lane_errors):
             _check(rc, allow=(OK, NOT_CONVERGED) if allow_not_converged else (OK,))
         shaped = [x.reshape(k, m.n) if k > 1 else x for x, k, m in zip(xs, self.nrhs, self.mats)]
         return {"x": shaped, "lanes": [st.as_dict() for st in lane_st], "total": tot.as_dict(),

@@ -686,6 +686,11 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     if (!A.has_jac) return fail(CEL_ERR_STATE, "cel_gpu_mat_set_jacobi has not been called");
     if (A.ncols != n) return fail(CEL_ERR_ARG, "PCG needs a square local block");
     if (prm.iter_max < 0) return fail(CEL_ERR_ARG, "iter_max < 0");
+    if (n == 0) {  // nothing to solve (the kernels assume at least one row)
+        stats = cel_pcg_stats{};
+        stats.status = CEL_OK;
+        return CEL_OK;
+    }
 
     Workspace& W = ws_of(A);
     const int ntiles = std::max(1, (n + kTileRows - 1) / kTileRows);

@@ -58,9 +58,19 @@ __global__ __launch_bounds__(kBlock) void k_jad_spmv(int n, int mmax, const int*
     }
 }
 
+// A block with no columns has no entries: y = 0 (or unchanged).  The tile
+// kernels are not launched for it -- their branch-free gathers read x[0].
+template <typename T>
+static int spmv_no_columns(const Mat& A, T* y, int accumulate)
+{
+    if (!accumulate) CEL_HIP(hipMemsetAsync(y, 0, sizeof(T) * (size_t)A.n, rt().stream));
+    return CEL_OK;
+}
+
 int launch_csr_spmv(const Mat& A, const double* x, double* y, int accumulate)
 {
     if (A.n == 0) return CEL_OK;
+    if (A.ncols == 0) return spmv_no_columns(A, y, accumulate);
     int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int G = resident_grid((const void*)k_csr_spmv<double>, ntiles);
     hipLaunchKernelGGL(k_csr_spmv<double>, dim3(G), dim3(kBlock), 0, rt().stream, A.n, A.xadj.p,
@@ -72,6 +82,7 @@ int launch_csr_spmv(const Mat& A, const double* x, double* y, int accumulate)
 int launch_csr_spmv_f32(const Mat& A, const float* x, float* y, int accumulate)
 {
     if (A.n == 0) return CEL_OK;
+    if (A.ncols == 0) return spmv_no_columns(A, y, accumulate);
     if (!A.vals32.p) return fail(CEL_ERR_STATE, "FP32 values not prepared");
     int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int G = resident_grid((const void*)k_csr_spmv<float>, ntiles);

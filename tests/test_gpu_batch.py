@@ -202,3 +202,13 @@ def test_batch_config4_workload(gpu):
             assert abs(out["lanes"][lane]["iters"] - single["iters"]) <= 1, (i, j)
             np.testing.assert_allclose(xs[lane], single["x"], rtol=0,
                                        atol=1e-6 * np.abs(single["x"]).max())
+
+
+def test_batch_rejects_mixed_precision(gpu):
+    """The batched solver is FP64: a mixed-precision request is an argument
+    error, raised (not reported as a lane outcome)."""
+    M = gpu.Matrix.poisson7(5, 5, 5)
+    M.set_jacobi()
+    batch = gpu.Batch([M])
+    with pytest.raises(gpu.CelError):
+        batch.solve([np.ones(M.n)], precision=gpu.PREC_MIXED)

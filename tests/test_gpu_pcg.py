@@ -260,3 +260,37 @@ def test_config3_mixed_384(gpu):
     assert true_rel <= 1e-10
     assert d_fp64 <= 1e-7
     assert abs(mixed["iters"] - fp64["iters"]) <= 0.05 * fp64["iters"]
+
+
+def test_zero_rhs_returns_at_once(gpu):
+    """b = 0, x0 = 0: the initial residual is 0, so the loop does not run
+    (the reference's norm2 > eps test with eps = tol * ||b|| = 0)."""
+    M = gpu.Matrix.poisson7(9, 8, 7)
+    M.set_jacobi()
+    out = M.pcg(np.zeros(M.n), tol=1e-10, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO)
+    assert out["status"] == 0 and out["iters"] == 0
+    np.testing.assert_array_equal(out["x"], np.zeros(M.n))
+
+
+def test_single_row_and_empty_systems(gpu):
+    """1 x 1: one iteration gives x = b / a exactly (Jacobi preconditioning
+    makes p = r / a, sigma = 1).  0 x 0: nothing to do, status 0."""
+    M = gpu.Matrix.from_csr([1, 2], [1], [4.0])
+    M.set_jacobi()
+    out = M.pcg(np.array([2.0]), tol=1e-12, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO)
+    assert out["status"] == 0 and out["iters"] == 1
+    assert out["x"][0] == 0.5
+    E = gpu.Matrix.from_csr([1], [], [])
+    assert E.n == 0
+    E.set_jacobi()
+    out = E.pcg(np.zeros(0), tol=1e-10, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO)
+    assert out["status"] == 0 and out["iters"] == 0
+
+
+def test_spmv_block_without_columns(gpu):
+    """A block with rows but no columns (a rank whose rows couple only to
+    neighbours): y = 0, or y unchanged when accumulating."""
+    M = gpu.Matrix.from_csr([1, 1, 1], [], [], ncols=0)
+    np.testing.assert_array_equal(M.spmv(np.zeros(0)), np.zeros(2))
+    y0 = np.array([3.0, -1.0])
+    np.testing.assert_array_equal(M.spmv(np.zeros(0), y0, accumulate=True), y0)
