@@ -87,11 +87,23 @@ def cpu_baseline(N, tol):
     same 256^3 system (x0 = 0, stop at ||r|| <= tol ||b||, the reference's
     absolute |alpha| test kept).  Returns the timing and the solution, the
     FP64 oracle the GPU solutions are compared with."""
+    # bind the worker threads to cores before the OpenMP runtime starts
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import oracle as O
 
     threads = len(os.sched_getaffinity(0))
     threads = max(2, min(threads, int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
     workers = threads - 1
+    cpu_model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     t0 = time.time()
     A = O.gen_poisson7(N, N, N)
     xt = np.random.default_rng(1).random(A.n)
@@ -103,6 +115,13 @@ def cpu_baseline(N, tol):
                      iter_max=20000, workers=workers)
     dt = time.time() - t0
     log(f"cpu baseline: {r['iters']} iterations in {dt:.1f}s ({dt / max(r['iters'], 1) * 1e3:.1f} ms/iter)")
+    # the reference's own thread count of the CPU-runnable case: 4 threads
+    # (3 computing workers), on a bounded sample of iterations
+    t0 = time.time()
+    r4 = O.pcg_jacobi(A, jac, b, x0_mode=O.X0_ZERO, eps=0.0, iter_max=60, workers=3)
+    dt4 = time.time() - t0
+    per4 = dt4 / max(r4["iters"], 1)
+    log(f"cpu baseline, 4 threads: {r4['iters']} iterations in {dt4:.1f}s ({per4 * 1e3:.1f} ms/iter)")
     return {
         "value": dt,
         "unit": "s",
@@ -110,9 +129,15 @@ def cpu_baseline(N, tol):
         "kind": "port",
         "sample": (f"the full FP64 solve of the {N}^3 system to {tol:g} relative residual, "
                    f"{r['iters']} Jacobi-PCG iterations (oracle/cel_oracle.c, reference op order, "
-                   f"{workers} workers + idle master thread), solve only"),
+                   f"{workers} workers + idle master thread, OMP_PROC_BIND=close "
+                   f"OMP_PLACES=cores), solve only"),
         "iters": r["iters"],
         "per_iter_ms": dt / max(r["iters"], 1) * 1e3,
+        "host_cpu": cpu_model,
+        "host_logical_cpus": os.cpu_count(),
+        "four_thread_point": {"threads": 4, "workers": 3, "iters": r4["iters"],
+                              "per_iter_ms": per4 * 1e3,
+                              "extrapolated_solve_s": per4 * r["iters"]},
     }, r["x"]
 
 
