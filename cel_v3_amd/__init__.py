@@ -17,13 +17,19 @@ LIB_PATH = os.path.join(_HERE, "libcel_gpu.so")
 
 OK, NOT_CONVERGED = 0, 1
 ERR = {-1: "ARG", -2: "HIP", -3: "NO_DEVICE", -4: "BAD_MATRIX", -5: "BAD_DIAG", -6: "BREAKDOWN",
-       -7: "NONFINITE", -8: "OOM", -9: "REF_UNDEFINED", -10: "STATE"}
+       -7: "NONFINITE", -8: "OOM", -9: "REF_UNDEFINED", -10: "STATE", -11: "FILE"}
+ERR_BAD_MATRIX, ERR_REF_UNDEFINED, ERR_FILE = -4, -9, -11
 FMT_CSR, FMT_JAD, FMT_COO = 1, 2, 3
 MEM_HOST, MEM_DEVICE = 0, 1
 X0_ONE, X0_ZERO, X0_USER = 0, 1, 2
 TOL_ABS, TOL_REL_B, TOL_REL_R0 = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 JAD_SPARSKIT, JAD_CEL_COMPAT = 0, 1
+MM_COORDINATE, MM_ARRAY = 0, 1
+MM_REAL, MM_COMPLEX, MM_PATTERN = 0, 1, 2
+MM_GENERAL, MM_SYMMETRIC, MM_HERMITIAN, MM_SKEW_SYMMETRIC = 0, 1, 2, 3
+MM_AS_STORED, MM_EXPAND_SYMMETRIC = 0, 1
+SORT_ROW_COL, SORT_CEL_COMPAT = 0, 1
 
 # every entry point declared in include/cel_gpu.h
 EXPORTS = [
@@ -40,6 +46,8 @@ EXPORTS = [
     "cel_gpu_pcg_solve", "cel_gpu_cg_adjoint", "cel_gpu_diffusion7_kappa_grad",
     "cel_gpu_mat_add_halo_block", "cel_gpu_stream_copy_time", "cel_gpu_batch_create",
     "cel_gpu_batch_destroy", "cel_gpu_batch_tune", "cel_gpu_batch_info", "cel_gpu_batch_solve",
+    "cel_gpu_mm_info", "cel_gpu_mm_read", "cel_gpu_coo_sort", "cel_gpu_coo_row_offsets",
+    "cel_gpu_mat_create_coo_sorted",
 ]
 
 
@@ -188,6 +196,39 @@ class Matrix:
                                             rows.size, _ptr(rows), _ptr(cols), _ptr(vals),
                                             index_base, row_offset, col_offset))
         return cls(h)
+
+    @classmethod
+    def from_coo_sorted(cls, nrows, rows, cols, vals, ncols=None, mode=SORT_CEL_COMPAT,
+                        index_base=1, row_offset=0, col_offset=0):
+        """Sort (as coo_sort), count rows and build the matrix in HBM.  rows,
+        cols, vals are host arrays or DeviceArrays (all of one kind)."""
+        h = _vp()
+        nc = nrows if ncols is None else ncols
+        if isinstance(rows, DeviceArray):
+            _check(lib().cel_gpu_mat_create_coo_sorted(C.byref(h), nrows, nc, rows.n, rows.ptr,
+                                                       cols.ptr, vals.ptr, index_base, row_offset,
+                                                       col_offset, mode, MEM_DEVICE))
+        else:
+            rows, cols, vals = _i32(rows), _i32(cols), _f64(vals)
+            _check(lib().cel_gpu_mat_create_coo_sorted(C.byref(h), nrows, nc, rows.size, _ptr(rows),
+                                                       _ptr(cols), _ptr(vals), index_base, row_offset,
+                                                       col_offset, mode, MEM_HOST))
+        return cls(h)
+
+    @classmethod
+    def from_mm(cls, path, mode=SORT_CEL_COMPAT, expand_symmetric=False):
+        """Matrix Market file -> matrix: read on the host (as the reference),
+        then one upload, sort, row count and build in HBM."""
+        m = mm_read(path, expand_symmetric=expand_symmetric)
+        if m["field"] != MM_REAL:
+            raise CelError(-1, "from_mm needs a real-valued file")
+        d = [DeviceArray.from_host(m["rows"]), DeviceArray.from_host(m["cols"]),
+             DeviceArray.from_host(m["vals"])]
+        try:
+            return cls.from_coo_sorted(m["nrows"], *d, ncols=m["ncols"], mode=mode)
+        finally:
+            for a in d:
+                a.free()
 
     @classmethod
     def poisson7(cls, nx, ny, nz):
@@ -466,6 +507,57 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- Matrix Market and COO order
+def mm_info(path):
+    """Header and size line of a Matrix Market file (host only)."""
+    v = [C.c_int() for _ in range(6)]
+    _check(lib().cel_gpu_mm_info(os.fsencode(path), *[C.byref(x) for x in v]))
+    return dict(zip(("rep", "field", "symm", "nrows", "ncols", "nnz"), (x.value for x in v)))
+
+
+def mm_read(path, expand_symmetric=False):
+    """Coordinate entries in file order, 1-based as written (host only).
+    Pattern files give vals of zeros (the library leaves them untouched)."""
+    info = mm_info(path)
+    cap = max(info["nnz"] * (2 if expand_symmetric else 1), 1)
+    rows, cols = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+    vals = np.zeros(cap, np.float64)
+    got = C.c_int()
+    _check(lib().cel_gpu_mm_read(os.fsencode(path), MM_EXPAND_SYMMETRIC if expand_symmetric else
+                                 MM_AS_STORED, cap, _ptr(rows), _ptr(cols), _ptr(vals),
+                                 C.byref(got)))
+    k = got.value
+    return dict(info, rows=rows[:k].copy(), cols=cols[:k].copy(), vals=vals[:k].copy())
+
+
+def coo_sort(nrows, rows, cols, vals, ncols=None, mode=SORT_CEL_COMPAT, index_base=1,
+             row_offset=0, col_offset=0, allow_undefined=False):
+    """Sorts in HBM.  Host arrays: returns (status, rows, cols, vals) as new
+    arrays.  DeviceArrays: sorted in place, returns (status, rows, cols, vals)
+    with the same DeviceArrays.  status is 0, or ERR_REF_UNDEFINED with
+    allow_undefined (arrays then sorted by row only)."""
+    nc = nrows if ncols is None else ncols
+    allow = (OK, ERR_REF_UNDEFINED) if allow_undefined else (OK,)
+    if isinstance(rows, DeviceArray):
+        rc = _check(lib().cel_gpu_coo_sort(nrows, nc, rows.n, rows.ptr, cols.ptr, vals.ptr,
+                                           index_base, row_offset, col_offset, mode, MEM_DEVICE),
+                    allow)
+        return rc, rows, cols, vals
+    r, c, v = _i32(rows).copy(), _i32(cols).copy(), _f64(vals).copy()
+    rc = _check(lib().cel_gpu_coo_sort(nrows, nc, r.size, _ptr(r), _ptr(c), _ptr(v), index_base,
+                                       row_offset, col_offset, mode, MEM_HOST), allow)
+    return rc, r, c, v
+
+
+def coo_row_offsets(nrows, rows, index_base=1, row_offset=0):
+    """cel_sp_mat_to_csr's xadj (nrows+1, starting at index_base)."""
+    r = _i32(rows)
+    xadj = np.zeros(nrows + 1, np.int32)
+    _check(lib().cel_gpu_coo_row_offsets(nrows, r.size, _ptr(r), index_base, row_offset, _ptr(xadj),
+                                         MEM_HOST))
+    return xadj
 
 
 def stream_copy_time(nbytes, reps=20):

@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcel_gpu.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["capi.hip", "spmv.hip", "jad.hip", "pcg.hip", "batch.hip"]
+SOURCES = ["capi.hip", "spmv.hip", "jad.hip", "pcg.hip", "batch.hip", "coo.hip", "mmio.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -31,12 +31,19 @@ FLAGS = [
 ]
 
 
+CXX = os.environ.get("CXX", shutil.which("g++") or "g++")
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    if src.endswith(".cpp"):  # host-only sources (file parsing)
+        cmd = [CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr}")
+        raise RuntimeError(f"{cmd[0]} failed on {src}:\n{r.stderr}")
     if r.stderr.strip():
         # keep warnings visible but filter the known host-side notice about
         # the packed-fp32 target feature

@@ -41,6 +41,8 @@ extern "C" {
 #define CEL_ERR_OOM (-8)
 #define CEL_ERR_REF_UNDEFINED (-9) /* the reference's result is undefined here */
 #define CEL_ERR_STATE (-10)     /* call order (e.g. JAD requested before conversion) */
+#define CEL_ERR_FILE (-11)      /* file not readable, or not a Matrix Market file the
+                                   reference's reader accepts */
 
 /* ---- enumerations ------------------------------------------------------ */
 /* storage formats; values equal MV_CSR/MV_JAD/MV_COO of
@@ -159,6 +161,69 @@ int cel_gpu_mat_info(const cel_gpu_mat* A, int* nrows, int* ncols, int* nnz, int
                      int* nblocks);
 /* 1-based host copies of the local block (xadj n+1, cols/vals nnz). */
 int cel_gpu_mat_get_csr(const cel_gpu_mat* A, int* xadj, int* cols, double* vals);
+
+/* ---- Matrix Market files, COO ordering, COO -> CSR --------------------- */
+#define CEL_MM_COORDINATE 0
+#define CEL_MM_ARRAY 1
+#define CEL_MM_REAL 0
+#define CEL_MM_COMPLEX 1
+#define CEL_MM_PATTERN 2
+#define CEL_MM_GENERAL 0
+#define CEL_MM_SYMMETRIC 1
+#define CEL_MM_HERMITIAN 2
+#define CEL_MM_SKEW_SYMMETRIC 3
+#define CEL_MM_AS_STORED 0         /* the reference: entries exactly as in the file */
+#define CEL_MM_EXPAND_SYMMETRIC 1  /* also append (j,i) for every stored (i,j), i != j
+                                      (value negated for skew-symmetric); no
+                                      reference counterpart */
+
+/* mminfo (src/fbase/cel_mmio_module.f90:342-466): header and size line.
+ * Host only (no GPU call).  nnz is the stored entry count (array files:
+ * the reference's count for the symmetry).  Any output may be NULL. */
+int cel_gpu_mm_info(const char* path, int* rep, int* field, int* symm, int* nrows, int* ncols,
+                    int* nnz);
+/* cel_sp_mat_read_matrix_market_file + mmread
+ * (src/fspmt/cel_sp_mat_generate_module.f90:219-273, src/fbase/cel_mmio_module.f90:17-339):
+ * the coordinate entries in file order, indices as written (1-based, not
+ * range-checked, as the reference), real values as read.  Host only.
+ * Pattern files leave vals untouched (the reference leaves its values
+ * unset).  Array files and complex files of more than one entry return
+ * CEL_ERR_REF_UNDEFINED (the reference sets no indices for the first and
+ * overruns a one-element buffer for the second); 'integer' fields are
+ * rejected with CEL_ERR_FILE as the reference rejects them.  cap is the
+ * capacity of rows/cols/vals; *nnz_out receives the entries written. */
+int cel_gpu_mm_read(const char* path, int flags, int cap, int* rows, int* cols, double* vals,
+                    int* nnz_out);
+
+#define CEL_SORT_ROW_COL 0     /* stable by (row, col) */
+#define CEL_SORT_CEL_COMPAT 1  /* cel_sp_mat_format_sparse_31_sort, quirks included */
+
+/* cel_sp_mat_format_sparse_31_sort (src/fspmt/cel_sp_mat_module.f90:518-585),
+ * in place on host or device arrays, sorted in HBM.  Row i of the block is
+ * rows = i + index_base + row_offset (i < nrows), columns likewise with
+ * col_offset and ncols; anything else is CEL_ERR_BAD_MATRIX.
+ * CEL_SORT_CEL_COMPAT gives the reference's order entry for entry: a stable
+ * sort by row, then stable by column within each segment its loop closes,
+ * so the last row keeps its file order, a first entry whose row value is
+ * not 1 stays first, and a last row of one entry is column-sorted together
+ * with the row before it.  Where its loop reads past the arrays (fewer row
+ * changes than nrows-1, e.g. an empty row) the arrays are left sorted by
+ * row and CEL_ERR_REF_UNDEFINED is returned. */
+int cel_gpu_coo_sort(int nrows, int ncols, int nnz, int* rows, int* cols, double* vals,
+                     int index_base, int row_offset, int col_offset, int mode, int memspace);
+/* cel_sp_mat_to_csr (src/fspmt/cel_sp_mat_module.f90:430-456): xadj from
+ * row counts (entries are not moved), xadj[0] = index_base; nrows+1
+ * entries.  A row outside the block is CEL_ERR_BAD_MATRIX. */
+int cel_gpu_coo_row_offsets(int nrows, int nnz, const int* rows, int index_base, int row_offset,
+                            int* xadj, int memspace);
+/* Sort (as cel_gpu_coo_sort), count rows, and build the matrix, all in
+ * HBM: what the reference does between reading a COO and its CSR SpMV
+ * (src/felmsolver/cel_elmsolver_module.f90:372, cel_sp_mat_to_csr).  Rows
+ * that decrease after the sort (a CEL_SORT_CEL_COMPAT quirk on a last row
+ * of one entry) are CEL_ERR_BAD_MATRIX. */
+int cel_gpu_mat_create_coo_sorted(cel_gpu_mat** A, int nrows, int ncols, int nnz, const int* rows,
+                                  const int* cols, const double* vals, int index_base,
+                                  int row_offset, int col_offset, int mode, int memspace);
 
 /* ---- CSR -> JAD on the device ------------------------------------------ */
 /* Replaces cel_sp_mat_jad_from_sp_mat_local + cel_sp_mat_jad_dcsort

@@ -19,6 +19,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
 _REF = os.path.join(_HERE, "_ref", "libcel_base_ref.so")
+_REF_MM = os.path.join(_HERE, "_ref", "libcel_mm_ref.so")
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -81,6 +82,8 @@ def _declare(L):
                                   _i32p, _i32p, _i32p, _i32p, _f64p, C.POINTER(C.c_int), _i32p,
                                   _i32p, _i32p, _i32p, _f64p]
     L.orc_face_pack.argtypes = [C.c_int, _i32p, _f64p, _f64p]
+    L.orc_coo_sort_cel.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p]
+    L.orc_coo_sort_rowcol.argtypes = [C.c_int, _i32p, _i32p, _f64p]
     L.orc_sddmm_csr.argtypes = [C.c_int, _i32p, _i32p, C.c_int, _f64p, _f64p, _f64p]
     L.orc_diffusion7_kappa_grad.argtypes = [C.c_int] * 3 + [_f64p, _f64p, _f64p, _f64p]
 
@@ -289,6 +292,22 @@ def distr_split(vtxdist, my_rank, rows, cols, vals) -> dict:
     return dict(blocks=blocks, row_offset=roff.value, diagonal=diag[:n], n=n)
 
 
+def coo_sort_cel(nrows, rows, cols, vals):
+    """cel_sp_mat_format_sparse_31_sort restated (cel_oracle.h).  Returns
+    (status, rows, cols, vals) on copies: status 0, -1 (the reference reads
+    past its arrays: undefined) or -2 (the reference stops: row below 1)."""
+    r, c, v = _i32(rows).copy(), _i32(cols).copy(), _f64(vals).copy()
+    st = lib().orc_coo_sort_cel(int(nrows), r.size, r, c, v)
+    return st, r, c, v
+
+
+def coo_sort_rowcol(rows, cols, vals):
+    """Stable sort by (row, col) on copies."""
+    r, c, v = _i32(rows).copy(), _i32(cols).copy(), _f64(vals).copy()
+    lib().orc_coo_sort_rowcol(r.size, r, c, v)
+    return r, c, v
+
+
 def face_pack(perm, x):
     perm = _i32(perm)
     buf = np.empty(perm.size)
@@ -358,3 +377,48 @@ def ref_generate(nx, ny, nz, dx=1, dy=1, dz=1, rank=0, stencil=7):
     xt = np.ctypeslib.as_array(xvec.values, (xvec.length,)).copy()
     return dict(rows=rows, cols=cols, vals=vals, b=b, x_theory=xt, rows_num=coo.rows_num,
                 cols_num=coo.cols_num, block_rows_num=coo.block_rows_num, bxyz=dom.bxyz)
+
+
+# ---------------------------------------------------------------- reference reader and sort
+_mm = None
+
+
+def ref_mm_available() -> bool:
+    return os.path.exists(_REF_MM)
+
+
+def _ref_mm():
+    global _mm
+    if _mm is None:
+        L = C.CDLL(_REF_MM)
+        L.ref_mm_info.argtypes = [C.c_char_p, C.c_int] + [C.POINTER(C.c_int)] * 3
+        L.ref_mm_read.argtypes = [C.c_char_p, C.c_int, C.c_int, _i32p, _i32p, _f64p]
+        L.ref_coo_sort.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p]
+        _mm = L
+    return _mm
+
+
+def ref_mm_read(path):
+    """The reference's mminfo + mmread on a file (oracle/_ref, built from the
+    reference sources).  The reference STOPs the process on a malformed
+    file: call it only on files it accepts, or from a child process."""
+    b = os.fsencode(path)
+    r, c, z = C.c_int(), C.c_int(), C.c_int()
+    _ref_mm().ref_mm_info(b, len(b), C.byref(r), C.byref(c), C.byref(z))
+    nnz = z.value
+    rows = np.zeros(max(nnz, 1), np.int32)
+    cols = np.zeros(max(nnz, 1), np.int32)
+    vals = np.zeros(max(nnz, 1), np.float64)
+    got = _ref_mm().ref_mm_read(b, len(b), max(nnz, 1), rows, cols, vals)
+    if got < 0:
+        raise RuntimeError("reference reader refused")
+    return r.value, c.value, rows[:nnz], cols[:nnz], vals[:nnz]
+
+
+def ref_coo_sort(nrows, rows, cols, vals):
+    """The reference's cel_sp_mat_format_sparse_31_sort on copies.  Call it
+    only where coo_sort_cel returns 0: elsewhere the reference reads past
+    its arrays."""
+    r, c, v = _i32(rows).copy(), _i32(cols).copy(), _f64(vals).copy()
+    err = _ref_mm().ref_coo_sort(int(nrows), r.size, r, c, v)
+    return err, r, c, v

@@ -107,6 +107,80 @@ void orc_coo_to_csr(int nrows, int nnz, const int* rows, int row_offset, int* xa
     free(cnt);
 }
 
+void orc_stable_sort3(int* key, int* a, double* v, int n)
+{
+    if (n < 2) return;
+    /* bottom-up merge sort of an index permutation; ties take the left run */
+    int* idx = (int*)malloc((size_t)n * sizeof(int));
+    int* tmp = (int*)malloc((size_t)n * sizeof(int));
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    for (int w = 1; w < n; w *= 2) {
+        for (int lo = 0; lo < n; lo += 2 * w) {
+            int mid = lo + w < n ? lo + w : n;
+            int hi = lo + 2 * w < n ? lo + 2 * w : n;
+            int i = lo, j = mid, k = lo;
+            while (i < mid && j < hi) tmp[k++] = key[idx[i]] <= key[idx[j]] ? idx[i++] : idx[j++];
+            while (i < mid) tmp[k++] = idx[i++];
+            while (j < hi) tmp[k++] = idx[j++];
+        }
+        int* s = idx; idx = tmp; tmp = s;
+    }
+    int* ki = (int*)malloc((size_t)n * sizeof(int));
+    int* ai = (int*)malloc((size_t)n * sizeof(int));
+    double* vi = (double*)malloc((size_t)n * sizeof(double));
+    for (int i = 0; i < n; ++i) { ki[i] = key[idx[i]]; ai[i] = a[idx[i]]; vi[i] = v[idx[i]]; }
+    memcpy(key, ki, (size_t)n * sizeof(int));
+    memcpy(a, ai, (size_t)n * sizeof(int));
+    memcpy(v, vi, (size_t)n * sizeof(double));
+    free(ki); free(ai); free(vi); free(idx); free(tmp);
+}
+
+int orc_coo_sort_cel(int nrows, int nnz, int* rows, int* cols, double* vals)
+{
+    orc_stable_sort3(rows, cols, vals, nnz);                 /* :532-535 */
+    /* the row-sorted state, restored when the loop runs off the end */
+    size_t ni = (size_t)(nnz > 0 ? nnz : 1);
+    int* r0 = (int*)malloc(ni * sizeof(int));
+    int* c0 = (int*)malloc(ni * sizeof(int));
+    double* v0 = (double*)malloc(ni * sizeof(double));
+    memcpy(r0, rows, (size_t)nnz * sizeof(int));
+    memcpy(c0, cols, (size_t)nnz * sizeof(int));
+    memcpy(v0, vals, (size_t)nnz * sizeof(double));
+    int status = 0;
+    /* :544-575 with the reference's 1-based indices */
+    int cur_index = 1, last_row = 1, row_counter = 0, start_index = 1, end_index = 1;
+    while (row_counter < nrows - 1) {
+        if (cur_index > nnz) { status = -1; break; }         /* rows(nnz+1) */
+        int cur_row = rows[cur_index - 1];
+        if (last_row != cur_row || cur_index == nnz) {
+            if (last_row > cur_row) { status = -2; break; }  /* :555-558 */
+            if (cur_index == nnz) end_index = cur_index;
+            if (start_index < end_index)                     /* :562-567 */
+                orc_stable_sort3(cols + start_index - 1, rows + start_index - 1,
+                                 vals + start_index - 1, end_index - start_index + 1);
+            start_index = end_index + 1;
+            row_counter += 1;
+            last_row = cur_row;
+        }
+        end_index = cur_index;
+        cur_index += 1;
+    }
+    if (status != 0) {
+        memcpy(rows, r0, (size_t)nnz * sizeof(int));
+        memcpy(cols, c0, (size_t)nnz * sizeof(int));
+        memcpy(vals, v0, (size_t)nnz * sizeof(double));
+    }
+    free(r0); free(c0); free(v0);
+    return status;
+}
+
+void orc_coo_sort_rowcol(int nnz, int* rows, int* cols, double* vals)
+{
+    /* two stable passes: by column, then by row */
+    orc_stable_sort3(cols, rows, vals, nnz);
+    orc_stable_sort3(rows, cols, vals, nnz);
+}
+
 void orc_csr_diagonal(int n, const int* xadj, const int* cols, const double* vals,
                       int row_offset, double* diag)
 {

@@ -58,6 +58,33 @@ void orc_gen_diffusion7(int nx, int ny, int nz, const double* kappa,
  * counts of a row-sorted COO; entries are not moved; xadj(1) = 1. */
 void orc_coo_to_csr(int nrows, int nnz, const int* rows, int row_offset, int* xadj);
 
+/* Stable ascending sort of n entries by key[], carrying a[] and v[] (ties
+ * keep their input order).  The output of a stable sort is fixed by the
+ * keys and the input order alone, so this gives the result of the
+ * reference's cel_sp_mat_merge_sort (src/fspmt/cel_sp_mat_module.f90:620-701:
+ * merge on '<=' takes the left run, :706-749 (:727); bubble-sort leaves swap
+ * on '>' only, :588-618), whatever its recursion and leaf sizes. */
+void orc_stable_sort3(int* key, int* a, double* v, int n);
+
+/* cel_sp_mat_format_sparse_31_sort (src/fspmt/cel_sp_mat_module.f90:518-585),
+ * 1-based COO in place: a stable sort by row (:532-535), then the column
+ * loop (:544-575) restated statement by statement.  Its quirks follow from
+ * the loop: the segment of the last transition it handles is the last one
+ * column-sorted, so the last row keeps its row-sorted (file) order; when
+ * rows(1) != 1 the first entry is excluded from its row's column sort; a
+ * last row of one entry is sorted together with the row before it; with
+ * nrows <= 1 nothing is column-sorted.
+ * Returns 0, or -1 where the loop would read rows(nnz+1) (fewer row
+ * transitions than nrows-1: empty rows, nnz == 0): the reference reads past
+ * its arrays there and its result is undefined; the arrays are then left
+ * in the row-sorted state (column sorts already done by the loop are
+ * undone).  -2 where the reference stops with "rows not sorted" (a row
+ * index below 1), arrays likewise row-sorted. */
+int orc_coo_sort_cel(int nrows, int nnz, int* rows, int* cols, double* vals);
+
+/* Stable sort by (row, col): what the column loop above intends. */
+void orc_coo_sort_rowcol(int nnz, int* rows, int* cols, double* vals);
+
 /* Extract the diagonal of a CSR block whose global row of local row i is
  * i + row_offset and whose column indices are global (the "diagonal" array
  * captured in cel_sp_mat_distr_module.f90:197-207; last write wins). */

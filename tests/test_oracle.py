@@ -502,3 +502,124 @@ def test_face_difference_operator_matches_generated_matrices():
         ref = dA @ v
         got = delta_apply(shape, kappa, kp, face_coeffs(shape, kappa), v)
         assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+# ---------------------------------------------------------------- COO sort
+# Hand traces of cel_sp_mat_format_sparse_31_sort
+# (src/fspmt/cel_sp_mat_module.f90:518-585).  Values carry the input
+# position so every move is visible.  Each expected order was derived by
+# stepping the loop (cur_index, last_row, row_counter, start/end_index).
+SORT_TRACES = {
+    # all rows present, rows(1) == 1: rows 1..n-1 column-sorted, the last
+    # row keeps its file order (the loop stops at its first entry)
+    "last_row_unsorted": (3, [1, 1, 2, 2, 3, 3], [2, 1, 3, 1, 3, 1],
+                          0, [1, 1, 2, 2, 3, 3], [1, 2, 1, 3, 3, 1], [1, 0, 3, 2, 4, 5]),
+    # row 1 empty, so rows(1) = 2 != 1: a transition at index 1 closes
+    # [1,1]; the first entry is left out of its row's column sort
+    "first_entry_excluded": (3, [2, 2, 2, 3, 3], [3, 2, 1, 2, 1],
+                             0, [2, 2, 2, 3, 3], [3, 1, 2, 2, 1], [0, 2, 1, 3, 4]),
+    # last row of one entry: the transition at index nnz closes
+    # [start, nnz], sorting it by column together with the row before it
+    "single_last_row_merged": (3, [1, 1, 2, 2, 2, 3], [2, 1, 3, 1, 2, 2],
+                               0, [1, 1, 2, 2, 3, 2], [1, 2, 1, 2, 2, 3], [1, 0, 3, 4, 5, 2]),
+    # nrows == 1: the loop body never runs; rows sorted (stably) only
+    "one_row": (1, [1, 1, 1], [3, 1, 2], 0, [1, 1, 1], [3, 1, 2], [0, 1, 2]),
+    # empty middle row: the transition at index nnz (rows unchanged) closes
+    # the last row, which is then sorted too
+    "empty_middle_row": (3, [3, 1, 3, 1], [2, 2, 1, 1],
+                         0, [1, 1, 3, 3], [1, 2, 1, 2], [3, 1, 2, 0]),
+    # too few transitions: the loop would read rows(nnz+1)
+    "reads_past_end": (3, [1, 3, 1], [2, 1, 1], -1, None, None, None),
+    "no_entries": (2, [], [], -1, None, None, None),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SORT_TRACES))
+def test_coo_sort_hand_traces(name):
+    nrows, rows, cols, st_want, r_want, c_want, p_want = SORT_TRACES[name]
+    vals = np.arange(len(rows), dtype=np.float64)
+    st, r, c, v = O.coo_sort_cel(nrows, rows, cols, vals)
+    assert st == st_want
+    if st == 0:
+        np.testing.assert_array_equal(r, r_want)
+        np.testing.assert_array_equal(c, c_want)
+        np.testing.assert_array_equal(v, p_want)
+    else:  # left sorted by row (stable)
+        order = np.argsort(np.asarray(rows), kind="stable")
+        np.testing.assert_array_equal(v, np.arange(len(rows))[order])
+    if O.ref_mm_available() and st == 0:
+        err, r2, c2, v2 = O.ref_coo_sort(nrows, rows, cols, vals)
+        assert err == 0
+        np.testing.assert_array_equal(r2, r)
+        np.testing.assert_array_equal(c2, c)
+        np.testing.assert_array_equal(v2, v)
+
+
+def random_coo_case(rng, all_rows=False):
+    n = int(rng.integers(1, 12))
+    nnz = int(rng.integers(0, 60))
+    rows = rng.integers(1, n + 1, nnz)
+    if all_rows:
+        rows = np.concatenate([np.arange(1, n + 1), rows])
+        rng.shuffle(rows)
+    cols = rng.integers(1, n + 4, rows.size)
+    return n, rows.astype(np.int32), cols.astype(np.int32), rng.random(rows.size)
+
+
+@pytest.mark.skipif(not O.ref_mm_available(), reason="reference sort not built (oracle/_ref)")
+def test_coo_sort_matches_compiled_reference():
+    """The restatement against the reference's own routine, compiled from
+    its sources, on every defined random case (duplicates included; runs of
+    32+ entries take the merge path, shorter ones the bubble path)."""
+    rng = np.random.default_rng(21)
+    checked = undefined = 0
+    for t in range(1500):
+        n, rows, cols, vals = random_coo_case(rng, all_rows=t % 3 == 0)
+        st, r, c, v = O.coo_sort_cel(n, rows, cols, vals)
+        if st != 0:
+            undefined += 1
+            continue
+        err, r2, c2, v2 = O.ref_coo_sort(n, rows, cols, vals)
+        assert err == 0
+        np.testing.assert_array_equal(r, r2)
+        np.testing.assert_array_equal(c, c2)
+        np.testing.assert_array_equal(v, v2)
+        checked += 1
+    assert checked > 1000 and undefined > 100
+
+
+def test_coo_sort_rowcol_is_lexicographic_and_stable():
+    rng = np.random.default_rng(22)
+    for _ in range(200):
+        n, rows, cols, vals = random_coo_case(rng)
+        r, c, v = O.coo_sort_rowcol(rows, cols, vals)
+        order = np.lexsort((cols, rows))  # stable: ties keep input order
+        np.testing.assert_array_equal(r, rows[order])
+        np.testing.assert_array_equal(c, cols[order])
+        np.testing.assert_array_equal(v, vals[order])
+
+
+def test_coo_sort_cel_differs_from_rowcol_only_where_the_quirks_say():
+    """With every row present and rows(1) == 1, the reference's order equals
+    the full (row, col) sort except inside the last row, which keeps the
+    row-sorted (file) order; a last row of one entry changes nothing when
+    its column is the largest."""
+    rng = np.random.default_rng(23)
+    for _ in range(200):
+        n, rows, cols, vals = random_coo_case(rng, all_rows=True)
+        if n < 2:
+            continue
+        st, r, c, v = O.coo_sort_cel(n, rows, cols, vals)
+        assert st == 0
+        r0, c0, v0 = O.coo_sort_rowcol(rows, cols, vals)
+        last = int((rows == n).sum())
+        head = rows.size - last
+        if last > 1:
+            np.testing.assert_array_equal(r[:head], r0[:head])
+            np.testing.assert_array_equal(c[:head], c0[:head])
+            file_order = np.flatnonzero(rows == n)
+            np.testing.assert_array_equal(v[head:], vals[file_order])
+        elif cols[rows == n][0] >= cols[rows == n - 1].max():
+            np.testing.assert_array_equal(r, r0)
+            np.testing.assert_array_equal(c, c0)
+            np.testing.assert_array_equal(v, v0)
