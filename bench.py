@@ -196,6 +196,76 @@ def batch_bench(cel, steps):
             "speedup": seq_ms / float(np.median(times)), "tune_s": tune_s}
 
 
+def sort_bench(cel, N, steps, warmup, skip_cpu):
+    """COO ordering in HBM: the N^3 7-point matrix in shuffled entry order
+    (numpy default_rng(7) permutation; the order a Matrix Market file need
+    not keep) sorted with the reference's order (CEL_SORT_CEL_COMPAT: row
+    sort + the column-loop segments) and with the full (row, col) order,
+    arrays resident in HBM, in place.  Each step restores the shuffled input
+    by a device copy outside the timed call.  Algorithmic bytes: the entry
+    arrays read once and written once, 2 x (4 + 4 + 8) = 32 B per entry."""
+    M = cel.Matrix.poisson7(N, N, N)
+    xadj, cols, vals = M.get_csr()
+    M.destroy()
+    rows = np.repeat(np.arange(1, N ** 3 + 1, dtype=np.int32), np.diff(xadj))
+    perm = np.random.default_rng(7).permutation(rows.size)
+    rows, cols, vals = rows[perm], cols[perm], vals[perm]
+    nnz = rows.size
+    src = [cel.DeviceArray.from_host(a) for a in (rows, cols, vals)]
+    work = [cel.DeviceArray(a.n, a.dtype) for a in src]
+    n = N ** 3
+    out = {"workload": f"poisson7 {N}^3 COO, shuffled, in place in HBM", "entries": int(nnz)}
+    for name, mode in (("cel_compat", cel.SORT_CEL_COMPAT), ("row_col", cel.SORT_ROW_COL)):
+        times = []
+        for k in range(warmup + steps):
+            for w, s_ in zip(work, src):
+                w.copy_from(s_)
+            cel.lib().cel_gpu_synchronize()
+            t0 = time.perf_counter()
+            cel.coo_sort(n, *work, mode=mode)
+            dt = time.perf_counter() - t0
+            if k >= warmup:
+                times.append(dt)
+        ms = float(np.median(times)) * 1e3
+        gbs = 32.0 * nnz / (ms * 1e-3) / 1e9
+        log(f"sort {name}: {ms:.2f} ms median of {steps}, {nnz / ms / 1e6:.2f} G entries/s, "
+            f"{gbs:.0f} GB/s algorithmic")
+        out[name] = {"ms": ms, "ms_all": [t * 1e3 for t in times], "entries_per_s": nnz / (ms * 1e-3),
+                     "algorithmic_GBs": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+    # correctness of the last (row, col) sort: the generator's CSR order
+    r_s, c_s = work[0].to_host(), work[1].to_host()
+    want_rows = np.repeat(np.arange(1, n + 1, dtype=np.int32), np.diff(xadj))
+    out["row_col_matches_generator"] = bool(np.array_equal(r_s, want_rows) and
+                                            np.array_equal(c_s, cols[np.argsort(perm)]))
+    if not skip_cpu:
+        out["cpu_baseline"] = sort_cpu_baseline()
+        log(f"cpu baseline (reference sort): {out['cpu_baseline']}")
+    return out
+
+
+def sort_cpu_baseline():
+    """The reference's own cel_sp_mat_format_sparse_31_sort (oracle/_ref,
+    built from its sources; one thread, as it runs) on a bounded sample: the
+    160^3 7-point matrix (28,518,400 entries) in the same shuffled form, about
+    15 s of CPU work.  The oracle's restatement is timed instead when the
+    reference build is absent."""
+    import oracle as O
+
+    A = O.gen_poisson7(160, 160, 160)
+    rows = np.repeat(np.arange(1, A.n + 1, dtype=np.int32), np.diff(A.xadj))
+    perm = np.random.default_rng(7).permutation(rows.size)
+    r, c, v = rows[perm], A.cols[perm], A.vals[perm]
+    kind = "reference" if O.ref_mm_available() else "port"
+    t0 = time.perf_counter()
+    if kind == "reference":
+        O.ref_coo_sort(A.n, r, c, v)
+    else:
+        O.coo_sort_cel(A.n, r, c, v)
+    dt = time.perf_counter() - t0
+    return {"value": rows.size / dt, "unit": "entries/s", "cores": 1, "kind": kind,
+            "seconds": dt, "sample": f"160^3 poisson7 COO, {rows.size} entries, shuffled"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,12 +276,13 @@ def main():
     ap.add_argument("--format", default="csr", choices=["csr", "jad"])
     ap.add_argument("--skip-cpu", action="store_true", help="do not time the CPU baseline")
     ap.add_argument("--only", default="all",
-                    choices=["all", "fp64", "mixed", "spmv", "profile", "batch"],
+                    choices=["all", "fp64", "mixed", "spmv", "profile", "batch", "sort"],
                     help="run one part only; 'profile' = one streaming copy of known size "
                          "(counter calibration) and FP64, FP64-with-halo and mixed solves "
                          "capped at --iters iterations, for hardware-counter runs; 'batch' = "
                          "the batched workload (64 variable-coefficient systems + 128^3 with 16 "
-                         "right-hand sides) against one-by-one solves")
+                         "right-hand sides) against one-by-one solves; 'sort' = COO ordering of "
+                         "the shuffled --size^3 matrix in HBM")
     ap.add_argument("--iters", type=int, default=20, help="iteration cap of --only profile")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -225,6 +296,10 @@ def main():
     if args.only == "batch":
         cel.init(0)
         print(json.dumps(batch_bench(cel, args.steps)))
+        return
+    if args.only == "sort":
+        cel.init(0)
+        print(json.dumps(sort_bench(cel, args.size, args.steps, args.warmup, args.skip_cpu)))
         return
 
     N = args.size

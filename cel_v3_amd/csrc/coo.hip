@@ -6,20 +6,22 @@
 // matrix build after it.
 //
 // The reference sorts on one CPU thread: a stable merge sort by row, then a
-// loop that walks the row-sorted entries and merge-sorts each segment it
-// closes by column.  Both stages are stable sorts of contiguous segments,
-// so on the GPU each becomes one stable LSD radix sort (rocPRIM through
-// hipCUB) over packed keys:
-//   CEL_SORT_ROW_COL    key = row << cb | col                 (one sort)
-//   CEL_SORT_CEL_COMPAT key = row                              (sort 1)
-//                       key = segment << cb | col, the input   (sort 2)
-//                       in the order sort 1 left it
-// where `segment` numbers the contiguous groups that the reference's loop
-// column-sorts; every entry after the last group it reaches is a group of
-// its own, so it keeps its place.  The groups follow from the row-sorted
-// rows alone (transitions, below), computed with two scans.  Radix sort is
-// stable, and a stable sort's output is fixed by its keys and its input
-// order, so the result is the reference's, entry for entry.
+// loop that walks the row-sorted entries and merge-sorts by column each
+// contiguous segment it closes.  Here:
+//   stage 1  one stable LSD radix sort by row (rocPRIM through hipCUB), the
+//            value and column travelling with the key (12 bytes), so no
+//            entry is ever fetched from a random place;
+//   groups   the contiguous groups stage 2 sorts: every row run
+//            (CEL_SORT_ROW_COL), or the segments the reference's loop
+//            closes, every entry after the last of them a group of its own
+//            (CEL_SORT_CEL_COMPAT; transitions below), found with scans;
+//   stage 2  a stable sort by column inside each group: one thread per
+//            group of <= 32 entries (rank counting in LDS), one workgroup per
+//            group of <= 4096 (bitonic sort of unique (column, position) keys
+//            in LDS), and for anything longer one stable radix sort by
+//            (group, column) over all entries.
+// Every step is stable, and a stable sort's output is fixed by its keys and
+// its input order, so the result is the reference's, entry for entry.
 //
 // The reference's loop, restated on 0-based positions p of the row-sorted
 // entries (row r[p], 0-based local; R = r + row_offset + 1 is the value the
@@ -40,6 +42,8 @@ namespace cel {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kSmall = 32;     // groups up to this length: one thread each
+constexpr int kMedium = 4096;  // up to this length: one workgroup each, in LDS
 
 int grid_for(long long n)
 {
@@ -55,27 +59,58 @@ int bits_for(unsigned long long v)  // bits to hold 0..v
     return b;
 }
 
-// Local 0-based row/col of every entry, range check, packed keys, identity.
-__global__ void __launch_bounds__(kThreads)
-k_coo_keys(long long nnz, const int* __restrict__ rows, const int* __restrict__ cols, int rbase,
-           int nrows, int cbase, int ncols, int cb, int mode, uint64_t* __restrict__ key,
-           uint32_t* __restrict__ rkey, int* __restrict__ idx, int* __restrict__ bad)
+// What travels with the row key through the stage-1 radix sort: the value's
+// bits and the local column (12 bytes).
+struct Pay {
+    uint32_t lo, hi, col;
+};
+
+__device__ inline double pay_val(const Pay& p)
 {
-    long long stride = (long long)gridDim.x * blockDim.x;
+    return __longlong_as_double((long long)(((unsigned long long)p.hi << 32) | p.lo));
+}
+
+#define CEL_GRID_STRIDE(i, n)                                                     \
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
+         i += (long long)gridDim.x * blockDim.x)
+
+// Local 0-based row (the stage-1 key) and payload of every entry; range check.
+__global__ void __launch_bounds__(kThreads)
+k_pack(long long nnz, const int* __restrict__ rows, const int* __restrict__ cols,
+       const double* __restrict__ vals, int rbase, int nrows, int cbase, int ncols,
+       uint32_t* __restrict__ rk, Pay* __restrict__ pay, int* __restrict__ bad)
+{
     int any = 0;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += stride) {
+    CEL_GRID_STRIDE(i, nnz) {
         long long r = (long long)rows[i] - rbase;
         long long c = (long long)cols[i] - cbase;
         bool ok = r >= 0 && r < nrows && c >= 0 && c < ncols;
         any |= !ok;
-        uint32_t ru = ok ? (uint32_t)r : 0u, cu = ok ? (uint32_t)c : 0u;
-        if (mode == CEL_SORT_ROW_COL)
-            key[i] = ((uint64_t)ru << cb) | cu;
-        else
-            rkey[i] = ru;
-        idx[i] = (int)i;
+        unsigned long long b = (unsigned long long)__double_as_longlong(vals[i]);
+        rk[i] = ok ? (uint32_t)r : 0u;
+        pay[i] = Pay{(uint32_t)b, (uint32_t)(b >> 32), ok ? (uint32_t)c : 0u};
     }
     if (any) atomicOr(bad, 1);
+}
+
+// Entries back to (rows, cols, vals) in the order they stand.
+__global__ void __launch_bounds__(kThreads)
+k_unpack(long long nnz, const uint32_t* __restrict__ rk, const Pay* __restrict__ pay, int rbase,
+         int cbase, int* rows, int* cols, double* vals)
+{
+    CEL_GRID_STRIDE(p, nnz) {
+        Pay q = pay[p];
+        rows[p] = (int)rk[p] + rbase;
+        cols[p] = (int)q.col + cbase;
+        vals[p] = pay_val(q);
+    }
+}
+
+// CEL_SORT_ROW_COL: every row run is one group; mark[q] = q closes a group.
+__global__ void __launch_bounds__(kThreads)
+k_row_ends(long long nnz, const uint32_t* __restrict__ r, int* __restrict__ mark)
+{
+    CEL_GRID_STRIDE(q, nnz) mark[q] = q == nnz - 1 || r[q + 1] != r[q];
 }
 
 // t[p]: is p a transition of the reference's column loop.
@@ -83,8 +118,7 @@ __global__ void __launch_bounds__(kThreads)
 k_transitions(long long nnz, const uint32_t* __restrict__ r, long long row_offset,
               int* __restrict__ t)
 {
-    long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += stride) {
+    CEL_GRID_STRIDE(p, nnz) {
         int v;
         if (p == 0)
             v = nnz == 1 || ((long long)r[0] + row_offset) != 0;
@@ -99,20 +133,18 @@ __global__ void __launch_bounds__(kThreads)
 k_kth(long long nnz, const int* __restrict__ t, const int* __restrict__ ord, int K,
       int* __restrict__ pk)
 {
-    long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += stride)
-        if (t[p] && ord[p] == K) *pk = (int)p;
+    CEL_GRID_STRIDE(p, nnz) if (t[p] && ord[p] == K) *pk = (int)p;
 }
 
-// mark[q] = 1 when q closes a handled segment or lies after the last one.
+// CEL_SORT_CEL_COMPAT: mark[q] = 1 when q closes a segment the loop sorts,
+// or lies after the last one (each such entry is a group of its own).
 __global__ void __launch_bounds__(kThreads)
 k_group_ends(long long nnz, const int* __restrict__ t, const int* __restrict__ ord, int K,
              const int* __restrict__ pk, int* __restrict__ mark)
 {
     long long pK = *pk;
     long long endK = pK == nnz - 1 ? nnz - 1 : (pK > 1 ? pK - 1 : 0);
-    long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += stride) {
+    CEL_GRID_STRIDE(q, nnz) {
         bool handled_next = q + 1 <= nnz - 2 && t[q + 1] && ord[q + 1] <= K;  // e(q+1) = q
         bool last = q == nnz - 1 && t[q] && ord[q] <= K;                     // e = nnz-1
         bool first = q == 0 && t[0] && ord[0] <= K;                          // e(0) = 0
@@ -120,27 +152,175 @@ k_group_ends(long long nnz, const int* __restrict__ t, const int* __restrict__ o
     }
 }
 
-// key = group << cb | col of the entry now at p (group = exclusive count).
+// head[q] = q opens a group.
 __global__ void __launch_bounds__(kThreads)
-k_group_keys(long long nnz, const int* __restrict__ seg, const int* __restrict__ perm,
-             const int* __restrict__ cols, int cbase, int cb, uint64_t* __restrict__ key)
+k_heads(long long nnz, const int* __restrict__ mark, int* __restrict__ head)
 {
-    long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += stride)
-        key[p] = ((uint64_t)(uint32_t)seg[p] << cb) | (uint32_t)(cols[perm[p]] - cbase);
+    CEL_GRID_STRIDE(q, nnz) head[q] = q == 0 || mark[q - 1];
+}
+
+// starts[g] = first position of group g (ih = inclusive count of heads);
+// starts[G] = nnz.
+__global__ void __launch_bounds__(kThreads)
+k_starts(long long nnz, const int* __restrict__ head, const int* __restrict__ ih,
+         int* __restrict__ starts)
+{
+    CEL_GRID_STRIDE(q, nnz) {
+        if (head[q]) starts[ih[q] - 1] = (int)q;
+        if (q == nnz - 1) starts[ih[q]] = (int)nnz;
+    }
+}
+
+// Groups of at most kSmall entries, one thread each: the rank of entry j is
+// #{k : col_k < col_j} + #{k < j : col_k == col_j}, i.e. the stable order by
+// column, and entry j goes to slot rank.  A workgroup takes 256 consecutive
+// groups; when their entries fit (kTileCap) they are loaded into LDS and
+// written back with coalesced accesses, otherwise each thread works on its
+// group in global memory.  Longer groups are left in place here and listed
+// for k_group_medium (order of the list is free: groups are independent).
+constexpr int kTileCap = 2048;
+
+__global__ void __launch_bounds__(kThreads)
+k_group_small(int G, const int* __restrict__ starts, const uint32_t* __restrict__ rk,
+              const Pay* __restrict__ pay, int rbase, int cbase, int* rows, int* cols,
+              double* vals, int* __restrict__ longer, int* __restrict__ nlonger,
+              int* __restrict__ maxlen)
+{
+    __shared__ int st[kThreads + 1];
+    __shared__ uint32_t in_rk[kTileCap], out_rk[kTileCap];
+    __shared__ Pay in_pay[kTileCap], out_pay[kTileCap];
+    const int tid = threadIdx.x;
+    for (long long g0 = (long long)blockIdx.x * kThreads; g0 < G; g0 += (long long)gridDim.x * kThreads) {
+        const int ng = (int)min((long long)kThreads, G - g0);
+        for (int i = tid; i <= ng; i += kThreads) st[i] = starts[g0 + i];
+        __syncthreads();
+        const int base = st[0];
+        const int E = st[ng] - base;
+        const int g = (int)g0 + tid;
+        int s = 0, L = 0;
+        if (tid < ng) {
+            s = st[tid] - base;
+            L = st[tid + 1] - st[tid];
+            if (L > kSmall) {
+                longer[atomicAdd(nlonger, 1)] = g;
+                atomicMax(maxlen, L);
+            }
+        }
+        if (E <= kTileCap) {
+            const uint32_t* p32 = reinterpret_cast<const uint32_t*>(pay + base);
+            uint32_t* l32 = reinterpret_cast<uint32_t*>(in_pay);
+            for (int i = tid; i < 3 * E; i += kThreads) l32[i] = p32[i];
+            for (int i = tid; i < E; i += kThreads) in_rk[i] = rk[base + i];
+            __syncthreads();
+            if (tid < ng) {
+                if (L <= kSmall) {
+                    for (int j = 0; j < L; ++j) {
+                        const uint32_t cj = in_pay[s + j].col;
+                        int r = 0;
+                        for (int k = 0; k < L; ++k) {
+                            const uint32_t ck = in_pay[s + k].col;
+                            r += ck < cj || (ck == cj && k < j);
+                        }
+                        out_pay[s + r] = in_pay[s + j];
+                        out_rk[s + r] = in_rk[s + j];
+                    }
+                } else {  // k_group_medium rewrites this group later
+                    for (int j = 0; j < L; ++j) {
+                        out_pay[s + j] = in_pay[s + j];
+                        out_rk[s + j] = in_rk[s + j];
+                    }
+                }
+            }
+            __syncthreads();
+            for (int i = tid; i < E; i += kThreads) {
+                const Pay q = out_pay[i];
+                rows[base + i] = (int)out_rk[i] + rbase;
+                cols[base + i] = (int)q.col + cbase;
+                vals[base + i] = pay_val(q);
+            }
+        } else if (tid < ng && L <= kSmall) {
+            const int gs = base + s;
+            for (int j = 0; j < L; ++j) {
+                const uint32_t cj = pay[gs + j].col;
+                int r = 0;
+                for (int k = 0; k < L; ++k) {
+                    const uint32_t ck = pay[gs + k].col;
+                    r += ck < cj || (ck == cj && k < j);
+                }
+                const Pay q = pay[gs + j];
+                rows[gs + r] = (int)rk[gs + j] + rbase;
+                cols[gs + r] = (int)cj + cbase;
+                vals[gs + r] = pay_val(q);
+            }
+        }
+        __syncthreads();  // st and the tiles are reused by the next round
+    }
+}
+
+// Groups of kSmall+1 .. kMedium entries, one workgroup each: bitonic sort
+// in LDS of the unique keys (col << 32 | j), so equal columns keep their
+// order; then entry j goes to its sorted slot.
+__global__ void __launch_bounds__(kThreads)
+k_group_medium(const int* __restrict__ list, const int* __restrict__ starts,
+               const uint32_t* __restrict__ rk, const Pay* __restrict__ pay, int rbase, int cbase,
+               int* rows, int* cols, double* vals)
+{
+    __shared__ unsigned long long key[kMedium];
+    const int g = list[blockIdx.x];
+    const int s = starts[g];
+    const int L = starts[g + 1] - s;
+    int P = 1;
+    while (P < L) P <<= 1;
+    for (int i = threadIdx.x; i < P; i += kThreads)
+        key[i] = i < L ? ((unsigned long long)pay[s + i].col << 32) | (unsigned)i : ~0ULL;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P; i += kThreads) {
+                const int x = i ^ j;
+                if (x > i) {
+                    const unsigned long long a = key[i], b = key[x];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        key[i] = b;
+                        key[x] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int m = threadIdx.x; m < L; m += kThreads) {
+        const int j = (int)(key[m] & 0xffffffffULL);
+        const Pay q = pay[s + j];
+        rows[s + m] = (int)rk[s + j] + rbase;
+        cols[s + m] = (int)q.col + cbase;
+        vals[s + m] = pay_val(q);
+    }
+}
+
+// Fallback for a group longer than kMedium: key = group << cb | col over all
+// entries (group = ih - 1), sorted stably with positions as values.
+__global__ void __launch_bounds__(kThreads)
+k_group_keys(long long nnz, const int* __restrict__ ih, const Pay* __restrict__ pay, int cb,
+             uint64_t* __restrict__ key, int* __restrict__ idx)
+{
+    CEL_GRID_STRIDE(p, nnz) {
+        key[p] = ((uint64_t)(uint32_t)(ih[p] - 1) << cb) | pay[p].col;
+        idx[p] = (int)p;
+    }
 }
 
 __global__ void __launch_bounds__(kThreads)
-k_gather3(long long nnz, const int* __restrict__ perm, const int* __restrict__ rin,
-          const int* __restrict__ cin, const double* __restrict__ vin, int* __restrict__ rout,
-          int* __restrict__ cout, double* __restrict__ vout)
+k_gather_out(long long nnz, const int* __restrict__ perm, const uint32_t* __restrict__ rk,
+             const Pay* __restrict__ pay, int rbase, int cbase, int* rows, int* cols, double* vals)
 {
-    long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += stride) {
-        int s = perm[p];
-        rout[p] = rin[s];
-        cout[p] = cin[s];
-        vout[p] = vin[s];
+    CEL_GRID_STRIDE(p, nnz) {
+        const int j = perm[p];
+        const Pay q = pay[j];
+        rows[p] = (int)rk[j] + rbase;
+        cols[p] = (int)q.col + cbase;
+        vals[p] = pay_val(q);
     }
 }
 
@@ -148,22 +328,20 @@ k_gather3(long long nnz, const int* __restrict__ perm, const int* __restrict__ r
 __global__ void __launch_bounds__(kThreads)
 k_rows_monotone(long long nnz, const int* __restrict__ rows, int* __restrict__ bad)
 {
-    long long stride = (long long)gridDim.x * blockDim.x;
     int any = 0;
-    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x + 1; p < nnz; p += stride)
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x + 1; p < nnz;
+         p += (long long)gridDim.x * blockDim.x)
         any |= rows[p] < rows[p - 1];
     if (any) atomicOr(bad, 1);
 }
 
-// cel_sp_mat_to_csr: counts per row, then a scan.  cnt has nrows+1 slots,
-// cnt[0] = 0; entries outside [0, nrows) are counted nowhere (the caller
-// has range-checked them).
+// cel_sp_mat_to_csr: counts per row, then a scan.  cnt has nrows+1 slots;
+// rows outside [0, nrows) are counted nowhere (flagged by k_rows_in_range).
 __global__ void __launch_bounds__(kThreads)
 k_row_count(long long nnz, const int* __restrict__ rows, int rbase, int nrows,
             int* __restrict__ cnt)
 {
-    long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += stride) {
+    CEL_GRID_STRIDE(p, nnz) {
         long long r = (long long)rows[p] - rbase;
         if (r >= 0 && r < nrows) atomicAdd(&cnt[r + 1], 1);
     }
@@ -173,9 +351,8 @@ __global__ void __launch_bounds__(kThreads)
 k_rows_in_range(long long nnz, const int* __restrict__ rows, int rbase, int nrows,
                 int* __restrict__ bad)
 {
-    long long stride = (long long)gridDim.x * blockDim.x;
     int any = 0;
-    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += stride) {
+    CEL_GRID_STRIDE(p, nnz) {
         long long r = (long long)rows[p] - rbase;
         any |= r < 0 || r >= nrows;
     }
@@ -185,20 +362,19 @@ k_rows_in_range(long long nnz, const int* __restrict__ rows, int rbase, int nrow
 __global__ void __launch_bounds__(kThreads)
 k_fill_int(long long n, int* __restrict__ a, int v)
 {
-    long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) a[i] = v;
+    CEL_GRID_STRIDE(i, n) a[i] = v;
 }
 
 __global__ void __launch_bounds__(kThreads)
 k_sub_int(long long n, const int* a, int v, int* out)  // may run in place
 {
-    long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = a[i] - v;
+    CEL_GRID_STRIDE(i, n) out[i] = a[i] - v;
 }
 
 struct SortBufs {
-    DevBuf<int> rin, cin, idx, perm, a, b, c, flag;
-    DevBuf<double> vin;
+    DevBuf<uint32_t> rk0, rk1;
+    DevBuf<Pay> pay0, pay1;
+    DevBuf<int> a, b, c, starts, idx, perm, lists;
     DevBuf<uint64_t> k0, k1;
     DevBuf<unsigned char> temp;
 };
@@ -210,108 +386,127 @@ int temp_fit(SortBufs& S, size_t bytes)
     return S.temp.alloc(bytes ? bytes : 1);
 }
 
-// Sorts (rows, cols, vals) on the device into (ro, co, vo); inputs are
-// device copies owned by S (rin/cin/vin).  Returns CEL_OK or
-// CEL_ERR_REF_UNDEFINED (outputs then row-sorted).
+template <typename T>
+int d2h_int(T* host, const T* dev)
+{
+    CEL_HIP(hipMemcpyAsync(host, dev, sizeof(T), hipMemcpyDeviceToHost, rt().stream));
+    CEL_HIP(hipStreamSynchronize(rt().stream));
+    return CEL_OK;
+}
+
+// Sorts (ri, ci, vi) into (ro, co, vo), device arrays; the outputs may be
+// the inputs (the inputs are consumed by the first kernel).  Returns CEL_OK
+// or CEL_ERR_REF_UNDEFINED (outputs then in the row-sorted order).
 int sort_device(SortBufs& S, int nrows, int ncols, long long nnz, int rbase, int cbase,
-                long long row_offset, int mode, int* ro, int* co, double* vo)
+                long long row_offset, int mode, const int* ri, const int* ci, const double* vi,
+                int* ro, int* co, double* vo)
 {
     hipStream_t st = rt().stream;
     const int g = grid_for(nnz);
     const int n = (int)nnz;
-    const int cb = bits_for(ncols > 0 ? (unsigned long long)(ncols - 1) : 0ULL);
     const int rb = bits_for(nrows > 0 ? (unsigned long long)(nrows - 1) : 0ULL);
-    CEL_TRY(S.idx.alloc(nnz));
-    CEL_TRY(S.perm.alloc(nnz));
-    CEL_TRY(S.k0.alloc(nnz));
-    CEL_TRY(S.k1.alloc(nnz));
-    CEL_TRY(S.flag.alloc(2));
-    CEL_HIP(hipMemsetAsync(S.flag.p, 0, 2 * sizeof(int), st));
-    uint32_t* rk0 = reinterpret_cast<uint32_t*>(S.k0.p);
-    uint32_t* rk1 = reinterpret_cast<uint32_t*>(S.k1.p);
-    k_coo_keys<<<g, kThreads, 0, st>>>(nnz, S.rin.p, S.cin.p, rbase, nrows, cbase, ncols, cb, mode,
-                                        S.k0.p, rk0, S.idx.p, S.flag.p);
+    CEL_TRY(S.rk0.alloc(nnz));
+    CEL_TRY(S.rk1.alloc(nnz));
+    CEL_TRY(S.pay0.alloc(nnz));
+    CEL_TRY(S.pay1.alloc(nnz));
+    CEL_TRY(S.lists.alloc(4));  // [0] bad, [1] K-th position, [2] longer count, [3] max length
+    CEL_HIP(hipMemsetAsync(S.lists.p, 0, 4 * sizeof(int), st));
+    k_pack<<<g, kThreads, 0, st>>>(nnz, ri, ci, vi, rbase, nrows, cbase, ncols, S.rk0.p, S.pay0.p,
+                                    S.lists.p);
     CEL_HIP(hipGetLastError());
     int bad = 0;
-    CEL_HIP(hipMemcpyAsync(&bad, S.flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    CEL_HIP(hipStreamSynchronize(st));
+    CEL_TRY(d2h_int(&bad, S.lists.p));
     if (bad) return fail(CEL_ERR_BAD_MATRIX, "row or column index out of range");
 
+    // stage 1: stable by row, the payload carried along
     size_t bytes = 0;
-    if (mode == CEL_SORT_ROW_COL) {
-        const int eb = cb + rb > 0 ? cb + rb : 1;
-        CEL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, S.k0.p, S.k1.p, S.idx.p, S.perm.p,
-                                                   n, 0, eb, st));
-        CEL_TRY(temp_fit(S, bytes));
-        CEL_HIP(hipcub::DeviceRadixSort::SortPairs(S.temp.p, bytes, S.k0.p, S.k1.p, S.idx.p, S.perm.p,
-                                                   n, 0, eb, st));
-        k_gather3<<<g, kThreads, 0, st>>>(nnz, S.perm.p, S.rin.p, S.cin.p, S.vin.p, ro, co, vo);
-        CEL_HIP(hipGetLastError());
-        return CEL_OK;
-    }
-
-    // stage 1: stable by row
-    CEL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, rk0, rk1, S.idx.p, S.perm.p, n, 0,
-                                               rb > 0 ? rb : 1, st));
+    const int eb = rb > 0 ? rb : 1;
+    CEL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, S.rk0.p, S.rk1.p, S.pay0.p, S.pay1.p,
+                                               n, 0, eb, st));
     CEL_TRY(temp_fit(S, bytes));
-    CEL_HIP(hipcub::DeviceRadixSort::SortPairs(S.temp.p, bytes, rk0, rk1, S.idx.p, S.perm.p, n, 0,
-                                               rb > 0 ? rb : 1, st));
-    const int K = nrows - 1;
-    if (K <= 0) {  // the loop body never runs
-        k_gather3<<<g, kThreads, 0, st>>>(nnz, S.perm.p, S.rin.p, S.cin.p, S.vin.p, ro, co, vo);
+    CEL_HIP(hipcub::DeviceRadixSort::SortPairs(S.temp.p, bytes, S.rk0.p, S.rk1.p, S.pay0.p, S.pay1.p,
+                                               n, 0, eb, st));
+    auto unpack = [&]() -> int {
+        k_unpack<<<g, kThreads, 0, st>>>(nnz, S.rk1.p, S.pay1.p, rbase, cbase, ro, co, vo);
         CEL_HIP(hipGetLastError());
         return CEL_OK;
-    }
+    };
+
+    // groups that are column-sorted
     CEL_TRY(S.a.alloc(nnz));
     CEL_TRY(S.b.alloc(nnz));
     CEL_TRY(S.c.alloc(nnz));
-    k_transitions<<<g, kThreads, 0, st>>>(nnz, rk1, row_offset, S.a.p);
-    CEL_HIP(hipGetLastError());
-    CEL_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, S.a.p, S.b.p, n, st));
-    CEL_TRY(temp_fit(S, bytes));
-    CEL_HIP(hipcub::DeviceScan::InclusiveSum(S.temp.p, bytes, S.a.p, S.b.p, n, st));
-    int total = 0;
-    CEL_HIP(hipMemcpyAsync(&total, S.b.p + (nnz - 1), sizeof(int), hipMemcpyDeviceToHost, st));
-    CEL_HIP(hipStreamSynchronize(st));
-    if (total < K) {
-        k_gather3<<<g, kThreads, 0, st>>>(nnz, S.perm.p, S.rin.p, S.cin.p, S.vin.p, ro, co, vo);
+    int* mark = S.c.p;
+    if (mode == CEL_SORT_ROW_COL) {
+        k_row_ends<<<g, kThreads, 0, st>>>(nnz, S.rk1.p, mark);
         CEL_HIP(hipGetLastError());
-        CEL_HIP(hipStreamSynchronize(st));
-        return fail(CEL_ERR_REF_UNDEFINED,
-                    "the reference's column loop needs nrows-1 row transitions and finds " +
-                        std::to_string(total) + ": it reads past its arrays (empty rows)");
+    } else {
+        const int K = nrows - 1;
+        if (K <= 0) return unpack();  // the loop body never runs
+        k_transitions<<<g, kThreads, 0, st>>>(nnz, S.rk1.p, row_offset, S.a.p);
+        CEL_HIP(hipGetLastError());
+        CEL_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, S.a.p, S.b.p, n, st));
+        CEL_TRY(temp_fit(S, bytes));
+        CEL_HIP(hipcub::DeviceScan::InclusiveSum(S.temp.p, bytes, S.a.p, S.b.p, n, st));
+        int total = 0;
+        CEL_TRY(d2h_int(&total, S.b.p + (nnz - 1)));
+        if (total < K) {
+            CEL_TRY(unpack());
+            CEL_HIP(hipStreamSynchronize(st));
+            return fail(CEL_ERR_REF_UNDEFINED,
+                        "the reference's column loop needs nrows-1 row transitions and finds " +
+                            std::to_string(total) + ": it reads past its arrays (empty rows)");
+        }
+        k_kth<<<g, kThreads, 0, st>>>(nnz, S.a.p, S.b.p, K, S.lists.p + 1);
+        k_group_ends<<<g, kThreads, 0, st>>>(nnz, S.a.p, S.b.p, K, S.lists.p + 1, mark);
+        CEL_HIP(hipGetLastError());
     }
-    k_kth<<<g, kThreads, 0, st>>>(nnz, S.a.p, S.b.p, K, S.flag.p + 1);
-    k_group_ends<<<g, kThreads, 0, st>>>(nnz, S.a.p, S.b.p, K, S.flag.p + 1, S.c.p);
+    int* head = S.a.p;
+    int* ih = S.b.p;
+    k_heads<<<g, kThreads, 0, st>>>(nnz, mark, head);
     CEL_HIP(hipGetLastError());
-    CEL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, S.c.p, S.a.p, n, st));
+    CEL_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, head, ih, n, st));
     CEL_TRY(temp_fit(S, bytes));
-    CEL_HIP(hipcub::DeviceScan::ExclusiveSum(S.temp.p, bytes, S.c.p, S.a.p, n, st));
-    // stage 2: stable by (group, col) over the row-sorted order
-    k_group_keys<<<g, kThreads, 0, st>>>(nnz, S.a.p, S.perm.p, S.cin.p, cbase, cb, S.k0.p);
+    CEL_HIP(hipcub::DeviceScan::InclusiveSum(S.temp.p, bytes, head, ih, n, st));
+    CEL_TRY(S.starts.alloc(nnz + 1));
+    k_starts<<<g, kThreads, 0, st>>>(nnz, head, ih, S.starts.p);
     CEL_HIP(hipGetLastError());
-    const int sb = bits_for((unsigned long long)nnz);
-    CEL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, S.k0.p, S.k1.p, S.perm.p, S.idx.p,
-                                               n, 0, cb + sb, st));
-    CEL_TRY(temp_fit(S, bytes));
-    CEL_HIP(hipcub::DeviceRadixSort::SortPairs(S.temp.p, bytes, S.k0.p, S.k1.p, S.perm.p, S.idx.p,
-                                               n, 0, cb + sb, st));
-    k_gather3<<<g, kThreads, 0, st>>>(nnz, S.idx.p, S.rin.p, S.cin.p, S.vin.p, ro, co, vo);
-    CEL_HIP(hipGetLastError());
-    return CEL_OK;
-}
+    int G = 0;
+    CEL_TRY(d2h_int(&G, ih + (nnz - 1)));
 
-int copy_in(SortBufs& S, long long nnz, const int* rows, const int* cols, const double* vals,
-            int memspace)
-{
-    CEL_TRY(S.rin.alloc(nnz));
-    CEL_TRY(S.cin.alloc(nnz));
-    CEL_TRY(S.vin.alloc(nnz));
-    hipMemcpyKind k = memspace == CEL_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    hipStream_t st = rt().stream;
-    CEL_HIP(hipMemcpyAsync(S.rin.p, rows, sizeof(int) * nnz, k, st));
-    CEL_HIP(hipMemcpyAsync(S.cin.p, cols, sizeof(int) * nnz, k, st));
-    CEL_HIP(hipMemcpyAsync(S.vin.p, vals, sizeof(double) * nnz, k, st));
+    // stage 2: stable by column inside each group
+    int* longer = mark;  // the group marks are spent; G <= nnz slots
+    k_group_small<<<grid_for(G), kThreads, 0, st>>>(G, S.starts.p, S.rk1.p, S.pay1.p, rbase, cbase,
+                                                     ro, co, vo, longer, S.lists.p + 2,
+                                                     S.lists.p + 3);
+    CEL_HIP(hipGetLastError());
+    int nl[2] = {0, 0};
+    CEL_HIP(hipMemcpyAsync(nl, S.lists.p + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    CEL_HIP(hipStreamSynchronize(st));
+    if (nl[0] == 0) return CEL_OK;
+    if (nl[1] <= kMedium) {
+        k_group_medium<<<nl[0], kThreads, 0, st>>>(longer, S.starts.p, S.rk1.p, S.pay1.p, rbase,
+                                                    cbase, ro, co, vo);
+        CEL_HIP(hipGetLastError());
+        return CEL_OK;
+    }
+    // a group longer than the LDS sort holds: one stable radix sort by
+    // (group, column) over everything
+    const int cb = bits_for(ncols > 0 ? (unsigned long long)(ncols - 1) : 0ULL);
+    const int sb = bits_for((unsigned long long)G);
+    CEL_TRY(S.k0.alloc(nnz));
+    CEL_TRY(S.k1.alloc(nnz));
+    CEL_TRY(S.idx.alloc(nnz));
+    CEL_TRY(S.perm.alloc(nnz));
+    k_group_keys<<<g, kThreads, 0, st>>>(nnz, ih, S.pay1.p, cb, S.k0.p, S.idx.p);
+    CEL_HIP(hipGetLastError());
+    CEL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, S.k0.p, S.k1.p, S.idx.p, S.perm.p, n,
+                                               0, cb + sb, st));
+    CEL_TRY(temp_fit(S, bytes));
+    CEL_HIP(hipcub::DeviceRadixSort::SortPairs(S.temp.p, bytes, S.k0.p, S.k1.p, S.idx.p, S.perm.p,
+                                               n, 0, cb + sb, st));
+    k_gather_out<<<g, kThreads, 0, st>>>(nnz, S.perm.p, S.rk1.p, S.pay1.p, rbase, cbase, ro, co, vo);
+    CEL_HIP(hipGetLastError());
     return CEL_OK;
 }
 
@@ -375,27 +570,32 @@ extern "C" int cel_gpu_coo_sort(int nrows, int ncols, int nnz, int* rows, int* c
             return fail(CEL_ERR_REF_UNDEFINED, "no entries: the reference's column loop reads rows(1)");
         return CEL_OK;
     }
-    SortBufs S;
-    CEL_TRY(copy_in(S, nnz, rows, cols, vals, memspace));
     const int rbase = index_base + row_offset, cbase = index_base + col_offset;
+    hipStream_t st = rt().stream;
+    SortBufs S;
     int rc;
     if (memspace == CEL_MEM_DEVICE) {
-        rc = sort_device(S, nrows, ncols, nnz, rbase, cbase, row_offset, mode, rows, cols, vals);
+        rc = sort_device(S, nrows, ncols, nnz, rbase, cbase, row_offset, mode, rows, cols, vals, rows,
+                         cols, vals);
     } else {
-        DevBuf<int> ro, co;
-        DevBuf<double> vo;
-        CEL_TRY(ro.alloc(nnz));
-        CEL_TRY(co.alloc(nnz));
-        CEL_TRY(vo.alloc(nnz));
-        rc = sort_device(S, nrows, ncols, nnz, rbase, cbase, row_offset, mode, ro.p, co.p, vo.p);
+        DevBuf<int> dr, dc;
+        DevBuf<double> dv;
+        CEL_TRY(dr.alloc(nnz));
+        CEL_TRY(dc.alloc(nnz));
+        CEL_TRY(dv.alloc(nnz));
+        CEL_HIP(hipMemcpyAsync(dr.p, rows, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice, st));
+        CEL_HIP(hipMemcpyAsync(dc.p, cols, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice, st));
+        CEL_HIP(hipMemcpyAsync(dv.p, vals, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice, st));
+        rc = sort_device(S, nrows, ncols, nnz, rbase, cbase, row_offset, mode, dr.p, dc.p, dv.p, dr.p,
+                         dc.p, dv.p);
         if (rc == CEL_OK || rc == CEL_ERR_REF_UNDEFINED) {
-            hipStream_t st = rt().stream;
-            CEL_HIP(hipMemcpyAsync(rows, ro.p, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost, st));
-            CEL_HIP(hipMemcpyAsync(cols, co.p, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost, st));
-            CEL_HIP(hipMemcpyAsync(vals, vo.p, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost, st));
+            CEL_HIP(hipMemcpyAsync(rows, dr.p, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost, st));
+            CEL_HIP(hipMemcpyAsync(cols, dc.p, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost, st));
+            CEL_HIP(hipMemcpyAsync(vals, dv.p, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost, st));
         }
+        CEL_HIP(hipStreamSynchronize(st));
     }
-    CEL_HIP(hipStreamSynchronize(rt().stream));
+    CEL_HIP(hipStreamSynchronize(st));
     return rc;
 }
 
@@ -440,23 +640,34 @@ extern "C" int cel_gpu_mat_create_coo_sorted(cel_gpu_mat** A, int nrows, int nco
     hipStream_t st = rt().stream;
     const int rbase = index_base + row_offset, cbase = index_base + col_offset;
     SortBufs S;
-    DevBuf<int> ro, co, xadj;
+    DevBuf<int> ro, co, xadj, flag;
     DevBuf<double> vo;
     CEL_TRY(ro.alloc(nnz > 0 ? nnz : 1));
     CEL_TRY(co.alloc(nnz > 0 ? nnz : 1));
     CEL_TRY(vo.alloc(nnz > 0 ? nnz : 1));
     CEL_TRY(xadj.alloc((size_t)nrows + 1));
     if (nnz > 0) {
-        CEL_TRY(copy_in(S, nnz, rows, cols, vals, memspace));
-        CEL_TRY(sort_device(S, nrows, ncols, nnz, rbase, cbase, row_offset, mode, ro.p, co.p, vo.p));
+        const int* ri = rows;
+        const int* ci = cols;
+        const double* vi = vals;
+        if (memspace == CEL_MEM_HOST) {
+            CEL_HIP(hipMemcpyAsync(ro.p, rows, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice, st));
+            CEL_HIP(hipMemcpyAsync(co.p, cols, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice, st));
+            CEL_HIP(hipMemcpyAsync(vo.p, vals, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice, st));
+            ri = ro.p;
+            ci = co.p;
+            vi = vo.p;
+        }
+        CEL_TRY(sort_device(S, nrows, ncols, nnz, rbase, cbase, row_offset, mode, ri, ci, vi, ro.p,
+                            co.p, vo.p));
         // a CSR build needs rows that never decrease; the reference's sort
         // can leave a last row of one entry inside the row before it
-        CEL_HIP(hipMemsetAsync(S.flag.p, 0, sizeof(int), st));
-        k_rows_monotone<<<grid_for(nnz), kThreads, 0, st>>>(nnz, ro.p, S.flag.p);
+        CEL_TRY(flag.alloc(1));
+        CEL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), st));
+        k_rows_monotone<<<grid_for(nnz), kThreads, 0, st>>>(nnz, ro.p, flag.p);
         CEL_HIP(hipGetLastError());
         int bad = 0;
-        CEL_HIP(hipMemcpyAsync(&bad, S.flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
-        CEL_HIP(hipStreamSynchronize(st));
+        CEL_TRY(d2h_int(&bad, flag.p));
         if (bad)
             return fail(CEL_ERR_BAD_MATRIX, "rows decrease after the sort (the reference's sort puts a "
                                             "last row of one entry among the row before it by column)");

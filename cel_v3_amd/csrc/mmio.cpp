@@ -36,6 +36,11 @@
 #include <string>
 #include <vector>
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include "../../include/cel_gpu.h"
 
 namespace cel {
@@ -46,30 +51,65 @@ namespace {
 
 using cel::fail;
 
+// The file mapped into memory (read whole when it cannot be mapped); records
+// are lines, addressed by pointers, so parsing allocates nothing per entry.
 struct MMFile {
-    FILE* f = nullptr;
-    std::string line;  // current record, '\n' and a trailing '\r' removed
-    size_t pos = 0;    // list-directed cursor within line
+    const char* data = nullptr;
+    size_t size = 0;
+    bool mapped = false;
+    std::vector<char> copy;
+    const char* next = nullptr;  // start of the next record
+    const char* lb = nullptr;    // current record [lb, le), '\n' and a trailing '\r' removed
+    const char* le = nullptr;
+    const char* cur = nullptr;   // list-directed cursor within the record
     long lineno = 0;
 
-    ~MMFile() { if (f) std::fclose(f); }
+    ~MMFile()
+    {
+        if (mapped) munmap(const_cast<char*>(data), size);
+    }
+
+    int open(const char* path)
+    {
+        int fd = ::open(path, O_RDONLY);
+        if (fd < 0) return -1;
+        struct stat sb;
+        if (fstat(fd, &sb) == 0 && S_ISREG(sb.st_mode) && sb.st_size > 0) {
+            void* m = mmap(nullptr, (size_t)sb.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+            if (m != MAP_FAILED) {
+                data = static_cast<const char*>(m);
+                size = (size_t)sb.st_size;
+                mapped = true;
+                madvise(m, size, MADV_SEQUENTIAL);
+            }
+        }
+        if (!mapped) {  // empty or not mappable: read it whole
+            char buf[1 << 16];
+            ssize_t got;
+            while ((got = ::read(fd, buf, sizeof buf)) > 0) copy.insert(copy.end(), buf, buf + got);
+            data = copy.data();
+            size = copy.size();
+        }
+        ::close(fd);
+        next = data;
+        return 0;
+    }
 
     bool next_record()
     {
-        line.clear();
-        pos = 0;
-        int c;
-        bool any = false;
-        while ((c = std::fgetc(f)) != EOF) {
-            any = true;
-            if (c == '\n') break;
-            line.push_back((char)c);
-        }
-        if (!any) return false;
-        if (!line.empty() && line.back() == '\r') line.pop_back();
+        const char* end = data + size;
+        if (next >= end) return false;
+        lb = next;
+        const char* nl = static_cast<const char*>(std::memchr(lb, '\n', (size_t)(end - lb)));
+        le = nl ? nl : end;
+        next = nl ? nl + 1 : end;
+        if (le > lb && le[-1] == '\r') --le;
+        cur = lb;
         ++lineno;
         return true;
     }
+
+    std::string line() const { return std::string(lb, le); }
 };
 
 // Word j (0-based) of the first 1024 characters, split at ' ' only.
@@ -107,49 +147,52 @@ bool is_sep(char c) { return c == ' ' || c == '\t'; }
 // `first` starts a new record (a READ begins with one); `comma` tells that
 // the previous value was closed by a comma, so a second comma (or a comma
 // opening the READ) marks a null value.
-// 1: token in tok; 0: end of file; <0: a form this reader refuses.
-int next_value(MMFile& F, std::string& tok, bool& first, bool& comma)
+// 1: token at [tb, tb+tn); 0: end of file; <0: a form this reader refuses.
+int next_value(MMFile& F, const char*& tb, size_t& tn, bool& first, bool& comma)
 {
     for (;;) {
         if (first) {
             if (!F.next_record()) return 0;
             first = false;
         }
-        std::string& s = F.line;
-        size_t& i = F.pos;
-        while (i < s.size() && is_sep(s[i])) ++i;
-        if (i >= s.size()) {
+        const char*& i = F.cur;
+        const char* e = F.le;
+        while (i < e && is_sep(*i)) ++i;
+        if (i >= e) {
             if (!F.next_record()) return 0;
             continue;
         }
-        if (s[i] == ',') {
+        if (*i == ',') {
             if (comma) return -2;
             comma = true;
             ++i;
             continue;
         }
-        size_t b = i;
-        while (i < s.size() && !is_sep(s[i]) && s[i] != ',') ++i;
-        tok = s.substr(b, i - b);
-        if (tok.find('/') != std::string::npos) return -3;
-        if (tok.find('*') != std::string::npos) return -4;
+        const char* b = i;
+        while (i < e && !is_sep(*i) && *i != ',') ++i;
+        tb = b;
+        tn = (size_t)(i - b);
+        for (const char* c = b; c < i; ++c) {
+            if (*c == '/') return -3;
+            if (*c == '*') return -4;
+        }
         // a comma right after the value is its separator
-        size_t j = i;
-        while (j < s.size() && is_sep(s[j])) ++j;
-        comma = j < s.size() && s[j] == ',';
+        const char* j = i;
+        while (j < e && is_sep(*j)) ++j;
+        comma = j < e && *j == ',';
         if (comma) i = j + 1;
         return 1;
     }
 }
 
-bool parse_int(const std::string& t, long long& v)
+bool parse_int(const char* t, size_t n, long long& v)
 {
     size_t i = 0;
     bool neg = false;
-    if (i < t.size() && (t[i] == '+' || t[i] == '-')) neg = t[i++] == '-';
-    if (i >= t.size()) return false;
+    if (i < n && (t[i] == '+' || t[i] == '-')) neg = t[i++] == '-';
+    if (i >= n) return false;
     long long x = 0;
-    for (; i < t.size(); ++i) {
+    for (; i < n; ++i) {
         if (t[i] < '0' || t[i] > '9') return false;
         x = x * 10 + (t[i] - '0');
         if (x > (long long)INT_MAX + 1) return false;
@@ -159,38 +202,51 @@ bool parse_int(const std::string& t, long long& v)
 }
 
 // Fortran real input: [s] digits [. digits] [exp] | [s] . digits [exp],
-// exp = (E|D|Q)[s]digits | s digits; also Inf/Infinity/NaN.  Converted by
-// strtod (correct rounding) after mapping the exponent to C form.
-bool parse_real(const std::string& t, double& v)
+// exp = (E|D|Q)[s]digits | s digits; also Inf/Infinity/NaN.  Rewritten to
+// C form in a local buffer and converted by strtod (correct rounding).
+bool parse_real(const char* t, size_t n, double& v)
 {
-    std::string low;
-    for (char c : t) low.push_back((c >= 'A' && c <= 'Z') ? (char)(c - 'A' + 'a') : c);
-    size_t i = 0;
-    std::string c;
-    if (i < low.size() && (low[i] == '+' || low[i] == '-')) c.push_back(low[i++]);
-    std::string rest = low.substr(i);
-    if (rest == "inf" || rest == "infinity" || rest == "nan" ||
-        (rest.size() > 4 && rest.compare(0, 4, "nan(") == 0 && rest.back() == ')')) {
-        v = std::strtod((c + rest).c_str(), nullptr);
+    char small[96];
+    std::string big;
+    char* c = small;
+    if (n + 2 > sizeof small) {
+        big.resize(n + 2);
+        c = &big[0];
+    }
+    size_t i = 0, o = 0;
+    auto low = [&](size_t k) { char ch = t[k]; return (ch >= 'A' && ch <= 'Z') ? (char)(ch - 'A' + 'a') : ch; };
+    if (i < n && (t[i] == '+' || t[i] == '-')) c[o++] = t[i++];
+    // Inf / Infinity / NaN / NaN(...)
+    if (i < n && (low(i) == 'i' || low(i) == 'n')) {
+        std::string rest;
+        for (size_t k = i; k < n; ++k) rest.push_back(low(k));
+        bool ok = rest == "inf" || rest == "infinity" || rest == "nan" ||
+                  (rest.size() > 4 && rest.compare(0, 4, "nan(") == 0 && rest.back() == ')');
+        if (!ok) return false;
+        for (char ch : rest) c[o++] = ch;
+        c[o] = 0;
+        v = std::strtod(c, nullptr);
         return true;
     }
     size_t digits = 0;
-    while (i < low.size() && low[i] >= '0' && low[i] <= '9') { c.push_back(low[i++]); ++digits; }
-    if (i < low.size() && low[i] == '.') {
-        c.push_back(low[i++]);
-        while (i < low.size() && low[i] >= '0' && low[i] <= '9') { c.push_back(low[i++]); ++digits; }
+    while (i < n && t[i] >= '0' && t[i] <= '9') { c[o++] = t[i++]; ++digits; }
+    if (i < n && t[i] == '.') {
+        c[o++] = t[i++];
+        while (i < n && t[i] >= '0' && t[i] <= '9') { c[o++] = t[i++]; ++digits; }
     }
     if (digits == 0) return false;
-    if (i < low.size()) {
-        if (low[i] == 'e' || low[i] == 'd' || low[i] == 'q') ++i;
-        else if (low[i] != '+' && low[i] != '-') return false;
-        c.push_back('e');
-        if (i < low.size() && (low[i] == '+' || low[i] == '-')) c.push_back(low[i++]);
+    if (i < n) {
+        char l = low(i);
+        if (l == 'e' || l == 'd' || l == 'q') ++i;
+        else if (l != '+' && l != '-') return false;
+        c[o++] = 'e';
+        if (i < n && (t[i] == '+' || t[i] == '-')) c[o++] = t[i++];
         size_t ed = 0;
-        while (i < low.size() && low[i] >= '0' && low[i] <= '9') { c.push_back(low[i++]); ++ed; }
-        if (ed == 0 || i != low.size()) return false;
+        while (i < n && t[i] >= '0' && t[i] <= '9') { c[o++] = t[i++]; ++ed; }
+        if (ed == 0 || i != n) return false;
     }
-    v = std::strtod(c.c_str(), nullptr);
+    c[o] = 0;
+    v = std::strtod(c, nullptr);
     return true;
 }
 
@@ -205,15 +261,17 @@ int read_error(int got, const MMFile& F, const std::string& what)
 }
 
 // One list-directed READ of k integers and m reals.
-int read_list(MMFile& F, int k, long long* iv, int m, double* rv, const std::string& what)
+template <typename What>
+int read_list(MMFile& F, int k, long long* iv, int m, double* rv, What what)
 {
     bool first = true, comma = true;
-    std::string tok;
+    const char* tb = nullptr;
+    size_t tn = 0;
     for (int j = 0; j < k + m; ++j) {
-        int got = next_value(F, tok, first, comma);
-        if (got != 1) return read_error(got, F, what);
-        bool ok = j < k ? parse_int(tok, iv[j]) : parse_real(tok, rv[j - k]);
-        if (!ok) return read_error(-1, F, what + " ('" + tok + "')");
+        int got = next_value(F, tb, tn, first, comma);
+        if (got != 1) return read_error(got, F, what());
+        bool ok = j < k ? parse_int(tb, tn, iv[j]) : parse_real(tb, tn, rv[j - k]);
+        if (!ok) return read_error(-1, F, what() + " ('" + std::string(tb, tn) + "')");
     }
     return CEL_OK;
 }
@@ -221,12 +279,12 @@ int read_list(MMFile& F, int k, long long* iv, int m, double* rv, const std::str
 // mminfo: header, comments, size line.  Leaves F positioned after the size line.
 int read_header(MMFile& F, const char* path, Header& H)
 {
-    F.f = std::fopen(path, "r");
-    if (!F.f) return fail(CEL_ERR_FILE, std::string("cannot open ") + path + ": " + std::strerror(errno));
+    if (F.open(path) != 0)
+        return fail(CEL_ERR_FILE, std::string("cannot open ") + path + ": " + std::strerror(errno));
     if (!F.next_record()) return fail(CEL_ERR_FILE, "Premature end-of-file: no lines found");
-    std::vector<std::string> w = header_words(F.line);
-    if (w.size() < 5) return fail(CEL_ERR_FILE, "Invalid matrix header: " + F.line);
-    if (fit(w[0], 15, false) != "%%MatrixMarket") return fail(CEL_ERR_FILE, "Invalid matrix header: " + F.line);
+    std::vector<std::string> w = header_words(F.line());
+    if (w.size() < 5) return fail(CEL_ERR_FILE, "Invalid matrix header: " + F.line());
+    if (fit(w[0], 15, false) != "%%MatrixMarket") return fail(CEL_ERR_FILE, "Invalid matrix header: " + F.line());
     if (fit(w[1], 6, true) != "matrix") return fail(CEL_ERR_FILE, "Invalid matrix type: " + w[1]);
     std::string rep = fit(w[2], 10, true), field = fit(w[3], 7, true), symm = fit(w[4], 19, true);
     if (rep == "coordinate") H.rep = CEL_MM_COORDINATE;
@@ -246,23 +304,27 @@ int read_header(MMFile& F, const char* path, Header& H)
     // comments: '%' in column 1; the first other line is the size line
     do {
         if (!F.next_record()) return fail(CEL_ERR_FILE, "Premature end-of-file: no data lines found");
-    } while (!F.line.empty() && F.line[0] == '%');
-    size_t words = header_words(F.line).size();
+    } while (F.le > F.lb && F.lb[0] == '%');
+    size_t words = header_words(F.line()).size();
     size_t want = H.rep == CEL_MM_COORDINATE ? 3 : 2;
     if (words != want)
         return fail(CEL_ERR_FILE, "Size info inconsistent with representation: " + std::to_string(want) +
                                       " size descriptors needed, " + std::to_string(words) + " found");
     // the size line is read list-directed from its own record
-    std::string rec = F.line;
+    std::string rec = F.line();
     long long iv[3] = {0, 0, 0};
     {
-        MMFile S;  // read from the saved record without advancing F
-        S.line = rec;
-        std::string tok;
+        MMFile S;  // the saved record alone, so F stays where it is
+        S.data = rec.data();
+        S.size = rec.size();
+        S.next = S.data;
+        S.next_record();
+        const char* tb = nullptr;
+        size_t tn = 0;
         bool first = false, comma = true;
         for (size_t j = 0; j < want; ++j) {
-            int got = next_value(S, tok, first, comma);
-            if (got != 1 || !parse_int(tok, iv[j]))
+            int got = next_value(S, tb, tn, first, comma);
+            if (got != 1 || !parse_int(tb, tn, iv[j]))
                 return fail(CEL_ERR_FILE, "unreadable size line: " + rec);
         }
     }
@@ -324,8 +386,9 @@ extern "C" int cel_gpu_mm_read(const char* path, int flags, int cap, int* rows, 
     for (long long k = 0; k < H.nnz; ++k) {
         long long iv[2];
         double rv[2];
-        rc = read_list(F, nint, iv, nreal, rv, "entry " + std::to_string(k + 1) + " of " +
-                                                  std::to_string(H.nnz));
+        rc = read_list(F, nint, iv, nreal, rv, [&]() {
+            return "entry " + std::to_string(k + 1) + " of " + std::to_string(H.nnz);
+        });
         if (rc < 0) return rc;
         rows[k] = (int)iv[0];
         cols[k] = (int)iv[1];

@@ -207,3 +207,46 @@ def test_matrix_market_file_to_pcg(gpu, tmp_path):
     st, r0, c0, v0 = O.coo_sort_cel(n, m["rows"], m["cols"], m["vals"])
     Mc = gpu.Matrix.from_mm(str(path), expand_symmetric=True)
     _check_same(Mc.get_csr(), (O.coo_to_csr(n, r0), c0, v0))
+
+
+def test_long_groups_take_the_lds_and_radix_paths(gpu):
+    """Rows longer than one thread's sort (33..4096 entries: one workgroup
+    each) and longer than the LDS sort (> 4096: one radix sort over all
+    groups), with duplicate columns, against the oracle in both modes."""
+    rng = np.random.default_rng(37)
+    for lens in ((40, 3, 700, 5, 33, 4096, 2), (3, 5000, 7, 64), (9000,), (1, 1, 1, 35, 1)):
+        n = len(lens)
+        rows = np.repeat(np.arange(1, n + 1), lens).astype(np.int32)
+        order = rng.permutation(rows.size)
+        rows = rows[order]
+        cols = rng.integers(1, 300, rows.size).astype(np.int32)
+        vals = rng.normal(size=rows.size)
+        print(f"lengths {lens}", flush=True)
+        st0, r0, c0, v0 = O.coo_sort_cel(n, rows, cols, vals)
+        st, r, c, v = gpu.coo_sort(n, rows, cols, vals, ncols=300, allow_undefined=True)
+        assert st == (0 if st0 == 0 else gpu.ERR_REF_UNDEFINED)
+        _check_same((r, c, v), (r0, c0, v0))
+        r1, c1, v1 = O.coo_sort_rowcol(rows, cols, vals)
+        _, r, c, v = gpu.coo_sort(n, rows, cols, vals, ncols=300, mode=gpu.SORT_ROW_COL)
+        _check_same((r, c, v), (r1, c1, v1))
+
+
+def test_dense_short_rows_overflow_the_lds_tile(gpu):
+    """600 rows of 20..32 entries: 256 consecutive groups hold more entries
+    than one workgroup's LDS tile, so the per-thread path in global memory
+    sorts them; 200 rows of 3 entries around them take the tile path."""
+    rng = np.random.default_rng(38)
+    lens = np.concatenate([rng.integers(1, 4, 200), rng.integers(20, 33, 600),
+                           rng.integers(1, 4, 200)])
+    n = lens.size
+    rows = np.repeat(np.arange(1, n + 1), lens).astype(np.int32)
+    rows = rows[rng.permutation(rows.size)]
+    cols = rng.integers(1, 50, rows.size).astype(np.int32)
+    vals = rng.normal(size=rows.size)
+    st0, r0, c0, v0 = O.coo_sort_cel(n, rows, cols, vals)
+    st, r, c, v = gpu.coo_sort(n, rows, cols, vals, ncols=50)
+    assert st == st0 == 0
+    _check_same((r, c, v), (r0, c0, v0))
+    r1, c1, v1 = O.coo_sort_rowcol(rows, cols, vals)
+    _, r, c, v = gpu.coo_sort(n, rows, cols, vals, ncols=50, mode=gpu.SORT_ROW_COL)
+    _check_same((r, c, v), (r1, c1, v1))
