@@ -30,6 +30,7 @@ MM_REAL, MM_COMPLEX, MM_PATTERN = 0, 1, 2
 MM_GENERAL, MM_SYMMETRIC, MM_HERMITIAN, MM_SKEW_SYMMETRIC = 0, 1, 2, 3
 MM_AS_STORED, MM_EXPAND_SYMMETRIC = 0, 1
 SORT_ROW_COL, SORT_CEL_COMPAT = 0, 1
+ELM_GMRES, ELM_PCG = 0, 1
 
 # every entry point declared in include/cel_gpu.h
 EXPORTS = [
@@ -47,7 +48,11 @@ EXPORTS = [
     "cel_gpu_mat_add_halo_block", "cel_gpu_stream_copy_time", "cel_gpu_batch_create",
     "cel_gpu_batch_destroy", "cel_gpu_batch_tune", "cel_gpu_batch_info", "cel_gpu_batch_solve",
     "cel_gpu_mm_info", "cel_gpu_mm_read", "cel_gpu_coo_sort", "cel_gpu_coo_row_offsets",
-    "cel_gpu_mat_create_coo_sorted",
+    "cel_gpu_mat_create_coo_sorted", "cel_gpu_gmres_params_default", "cel_gpu_gmres_solve",
+    "cel_gpu_elm_create", "cel_gpu_elm_row_start", "cel_gpu_elm_insert", "cel_gpu_elm_reinsert",
+    "cel_gpu_elm_row_completed", "cel_gpu_elm_matrix_completed", "cel_gpu_elm_set_solver",
+    "cel_gpu_elm_solve", "cel_gpu_elm_resolve", "cel_gpu_elm_collect_solution", "cel_gpu_elm_stats",
+    "cel_gpu_elm_matrix", "cel_gpu_elm_destroy",
 ]
 
 
@@ -63,6 +68,11 @@ class PcgParams(C.Structure):
                 ("memspace", C.c_int), ("use_halo", C.c_int), ("ru_delta", C.c_double),
                 ("ru_restart", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int),
                 ("profile", C.c_int), ("alpha_stop", C.c_int)]
+
+
+class GmresParams(C.Structure):
+    _fields_ = [("eps", C.c_double), ("iter_max", C.c_int), ("restart", C.c_int),
+                ("x0_mode", C.c_int), ("memspace", C.c_int), ("format", C.c_int)]
 
 
 class PcgStats(C.Structure):
@@ -93,6 +103,10 @@ def lib():
         L.cel_gpu_last_error.restype = C.c_char_p
         L.cel_gpu_pcg_params_default.restype = None
         L.cel_gpu_pcg_params_default.argtypes = [C.POINTER(PcgParams)]
+        L.cel_gpu_gmres_params_default.restype = None
+        L.cel_gpu_gmres_params_default.argtypes = [C.POINTER(GmresParams)]
+        L.cel_gpu_elm_insert.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.cel_gpu_elm_reinsert.argtypes = [C.c_void_p, C.c_int, C.c_double]
         for name in EXPORTS:
             getattr(L, name)  # every declared symbol must be exported
         _lib = L
@@ -407,6 +421,31 @@ class Matrix:
             out["hist"] = hist[: st.iters + 1]
         return out
 
+    def gmres(self, b, x0=None, eps=1e-8, iter_max=5000, restart=3, x0_mode=X0_ONE, fmt=FMT_CSR,
+              allow_not_converged=True):
+        """Jacobi-GMRES (cel_gmresalg_2jacobi).  b/x0 host arrays or DeviceArrays."""
+        p = GmresParams()
+        lib().cel_gpu_gmres_params_default(C.byref(p))
+        p.eps, p.iter_max, p.restart, p.x0_mode, p.format = eps, iter_max, restart, x0_mode, fmt
+        st = PcgStats()
+        if isinstance(b, DeviceArray):
+            p.memspace = MEM_DEVICE
+            x = x0 if x0 is not None else DeviceArray(self.n)
+            if x0 is not None:
+                p.x0_mode = X0_USER
+            rc = lib().cel_gpu_gmres_solve(self.h, b.ptr, x.ptr, C.byref(p), C.byref(st))
+        else:
+            p.memspace = MEM_HOST
+            x = np.zeros(self.n) if x0 is None else _f64(x0).copy()
+            if x0 is not None:
+                p.x0_mode = X0_USER
+            rc = lib().cel_gpu_gmres_solve(self.h, _ptr(_f64(b)), _ptr(x), C.byref(p), C.byref(st))
+        _check(rc, allow=(OK, NOT_CONVERGED) if allow_not_converged else (OK,))
+        out = st.as_dict()
+        out["x"] = x
+        out["cycles"] = out["outer_iters"]
+        return out
+
     def adjoint(self, x, dLdx, want_dLda=True, tol=1e-12, tol_mode=TOL_REL_B, iter_max=10000,
                 alpha_stop=0, **kw):
         p = params(tol=tol, tol_mode=tol_mode, iter_max=iter_max, x0_mode=X0_ZERO,
@@ -558,6 +597,96 @@ def coo_row_offsets(nrows, rows, index_base=1, row_offset=0):
     _check(lib().cel_gpu_coo_row_offsets(nrows, r.size, _ptr(r), index_base, row_offset, _ptr(xadj),
                                          MEM_HOST))
     return xadj
+
+
+# ---------------------------------------------------------------- element solver API
+class Elm:
+    """cel_elmSolver* as a handle (cel_gpu_elm_*): insert rows, complete the
+    matrix, solve, collect the solution."""
+
+    def __init__(self, n, nnz_per_row=(7, 0)):
+        self.n = n
+        self.h = _vp()
+        nz = (C.c_int * 2)(*nnz_per_row)
+        _check(lib().cel_gpu_elm_create(C.byref(self.h), n, n, 1, nz))
+
+    def row_start(self, row):
+        _check(lib().cel_gpu_elm_row_start(self.h, row))
+
+    def insert(self, col, value):
+        _check(lib().cel_gpu_elm_insert(self.h, col, value))
+
+    def reinsert(self, col, value):
+        _check(lib().cel_gpu_elm_reinsert(self.h, col, value))
+
+    def row_completed(self):
+        _check(lib().cel_gpu_elm_row_completed(self.h))
+
+    def insert_rows(self, rows, cols, vals):
+        """Entries in order; a row is started whenever it changes."""
+        cur = None
+        for r, c, v in zip(rows, cols, vals):
+            if r != cur:
+                if cur is not None:
+                    self.row_completed()
+                self.row_start(int(r))
+                cur = r
+            self.insert(int(c), float(v))
+        if cur is not None:
+            self.row_completed()
+
+    def matrix_completed(self):
+        _check(lib().cel_gpu_elm_matrix_completed(self.h))
+
+    def set_solver(self, solver):
+        _check(lib().cel_gpu_elm_set_solver(self.h, solver))
+
+    def _run(self, fn, b):
+        ierr, nit, ftol = C.c_int(), C.c_int(), C.c_double()
+        rc = fn(self.h, _ptr(_f64(b)), C.byref(ierr), C.byref(nit), C.byref(ftol))
+        _check(rc, allow=(OK, NOT_CONVERGED))
+        return dict(rc=rc, iierr=ierr.value, niter=nit.value, ftol=ftol.value)
+
+    def solve(self, b):
+        return self._run(lib().cel_gpu_elm_solve, b)
+
+    def resolve(self, b):
+        return self._run(lib().cel_gpu_elm_resolve, b)
+
+    def solution(self):
+        x = np.empty(self.n)
+        _check(lib().cel_gpu_elm_collect_solution(self.h, _ptr(x)))
+        return x
+
+    def stats(self):
+        st = PcgStats()
+        _check(lib().cel_gpu_elm_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def matrix_csr(self):
+        """1-based (xadj, cols, vals) of the assembled (scaled) matrix."""
+        a = _vp()
+        _check(lib().cel_gpu_elm_matrix(self.h, C.byref(a)))
+        m = Matrix.__new__(Matrix)
+        m.h = a
+        n, nc, nnz, mm, nb = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(lib().cel_gpu_mat_info(a, C.byref(n), C.byref(nc), C.byref(nnz), C.byref(mm),
+                                      C.byref(nb)))
+        m.n, m.ncols, m.nnz, m.nblocks = n.value, nc.value, nnz.value, nb.value
+        out = m.get_csr()
+        m.h = None  # owned by the handle
+        return out
+
+    def destroy(self):
+        if self.h:
+            lib().cel_gpu_elm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def stream_copy_time(nbytes, reps=20):

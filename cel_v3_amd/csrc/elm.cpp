@@ -1,0 +1,212 @@
+// elm.cpp -- the element solver API (row-wise insertion, assembly, solves)
+// as a C ABI, one handle per system.  Host C++ over the library's own
+// entry points: the assembly is the COO sort + CSR build in HBM, the solves
+// are the GPU Jacobi-GMRES (the reference's method) or Jacobi-PCG.
+//
+// Replaces src/felmsolver/cel_elmsolver_module.f90:
+//   cel_elmSolverInitialization :103-153   cel_gpu_elm_create
+//   cel_elmSolverDatarowStart   :155-159   cel_gpu_elm_row_start
+//   cel_elmSolverDataInsertion  :161-172   cel_gpu_elm_insert
+//   cel_elmSolverDataReinsertion :174-187  cel_gpu_elm_reinsert
+//   cel_elmSolverDatarowCompleted :189-191 cel_gpu_elm_row_completed
+//   cel_elmSolverMatrixCompleted :193-439  cel_gpu_elm_matrix_completed
+//   cel_elmSolverDoIt           :441-651   cel_gpu_elm_solve
+//   cel_elmSolverReDoIt         :653-788   cel_gpu_elm_resolve
+//   cel_elmSolverCollectSolution :790-815  cel_gpu_elm_collect_solution
+//   cel_elmSolverDestroy        :817-877   cel_gpu_elm_destroy
+//   cel_elmSolverPerformance    :879-900   cel_gpu_elm_stats
+// With one GPU the handle holds the whole system (the reference gathers the
+// rows of a reduce group on its master rank and splits them by rank).
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/cel_gpu.h"
+
+namespace cel {
+int fail(int code, const std::string& msg);
+}
+using cel::fail;
+
+struct cel_gpu_elm {
+    int n = 0;                       // local rows = global rows on one GPU
+    int row = 0;                     // current row (DatarowStart)
+    bool has_row = false;
+    bool completed = false;
+    int solver = CEL_ELM_GMRES;
+    std::vector<int> rows, cols;     // inserted entries, global 1-based
+    std::vector<double> vals;
+    std::vector<long long> diag_at;  // last diagonal entry of each row (-1: none)
+    cel_gpu_mat* A = nullptr;
+    std::vector<double> x;
+    cel_pcg_stats last{};
+};
+
+extern "C" int cel_gpu_elm_create(cel_gpu_elm** E, int itot, int ibs, int ibl, const int* nnzero)
+{
+    if (!E || itot < 0 || ibs < 0) return fail(CEL_ERR_ARG, "bad arguments to cel_gpu_elm_create");
+    if (ibs != itot || ibl != 1)
+        return fail(CEL_ERR_ARG, "one GPU solves the whole system: ibs must equal itotSize and ibl be 1");
+    auto* h = new cel_gpu_elm();
+    h->n = ibs;
+    // the reference allocates (nnzero(1)+nnzero(2))*ibs entries; here the
+    // arrays grow past that instead of being overrun
+    if (nnzero && nnzero[0] >= 0 && nnzero[1] >= 0) {
+        const long long cap = (long long)(nnzero[0] + nnzero[1]) * ibs;
+        if (cap > 0 && cap < (1LL << 31)) {
+            h->rows.reserve((size_t)cap);
+            h->cols.reserve((size_t)cap);
+            h->vals.reserve((size_t)cap);
+        }
+    }
+    h->diag_at.assign((size_t)ibs, -1);
+    *E = h;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_elm_row_start(cel_gpu_elm* E, int row)
+{
+    if (!E) return fail(CEL_ERR_ARG, "null handle");
+    if (E->completed) return fail(CEL_ERR_STATE, "the matrix is already completed");
+    if (row < 1 || row > E->n) return fail(CEL_ERR_ARG, "row " + std::to_string(row) + " is outside 1.." + std::to_string(E->n));
+    E->row = row;
+    E->has_row = true;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_elm_insert(cel_gpu_elm* E, int col, double value)
+{
+    if (!E) return fail(CEL_ERR_ARG, "null handle");
+    if (E->completed) return fail(CEL_ERR_STATE, "the matrix is already completed");
+    if (!E->has_row) return fail(CEL_ERR_STATE, "no row started");
+    if (E->rows.size() >= (size_t)0x7fffffff) return fail(CEL_ERR_ARG, "more than 2^31-1 entries");
+    E->rows.push_back(E->row);
+    E->cols.push_back(col);
+    E->vals.push_back(value);
+    if (col == E->row) E->diag_at[(size_t)E->row - 1] = (long long)E->rows.size() - 1;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_elm_reinsert(cel_gpu_elm* E, int col, double value)
+{
+    if (!E) return fail(CEL_ERR_ARG, "null handle");
+    if (E->completed) return fail(CEL_ERR_STATE, "the matrix is already completed");
+    if (!E->has_row) return fail(CEL_ERR_STATE, "no row started");
+    if (col != E->row)
+        return fail(CEL_ERR_ARG, "cel_elmSolverDataReinsertion is not implemented for off-diagonal entries");
+    const long long at = E->diag_at[(size_t)E->row - 1];
+    if (at < 0)
+        return fail(CEL_ERR_REF_UNDEFINED, "no diagonal entry was inserted in row " + std::to_string(E->row));
+    E->vals[(size_t)at] = value;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_elm_row_completed(cel_gpu_elm* E)
+{
+    return E ? CEL_OK : fail(CEL_ERR_ARG, "null handle");
+}
+
+extern "C" int cel_gpu_elm_matrix_completed(cel_gpu_elm* E)
+{
+    if (!E) return fail(CEL_ERR_ARG, "null handle");
+    if (E->completed) return fail(CEL_ERR_STATE, "the matrix is already completed");
+    int rc = cel_gpu_init(0);
+    if (rc < 0) return rc;
+    const int nnz = (int)E->rows.size();
+    // :372 the reference's sort, then to_csr and the build (one rank: every
+    // entry is in the local block), :386-419 diagonal scaling and the
+    // Jacobi vector of the scaled matrix
+    rc = cel_gpu_mat_create_coo_sorted(&E->A, E->n, E->n, nnz, E->rows.data(), E->cols.data(),
+                                       E->vals.data(), 1, 0, 0, CEL_SORT_CEL_COMPAT, CEL_MEM_HOST);
+    if (rc < 0) return rc;
+    rc = cel_gpu_mat_scale_diagonal(E->A);
+    if (rc < 0) return rc;
+    rc = cel_gpu_mat_set_jacobi(E->A, 1);
+    if (rc < 0) return rc;
+    E->completed = true;
+    // :431-434 the inserted arrays are released
+    std::vector<int>().swap(E->rows);
+    std::vector<int>().swap(E->cols);
+    std::vector<double>().swap(E->vals);
+    E->x.assign((size_t)E->n, 0.0);
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_elm_set_solver(cel_gpu_elm* E, int solver)
+{
+    if (!E || (solver != CEL_ELM_GMRES && solver != CEL_ELM_PCG))
+        return fail(CEL_ERR_ARG, "bad arguments to cel_gpu_elm_set_solver");
+    E->solver = solver;
+    return CEL_OK;
+}
+
+static int elm_run(cel_gpu_elm* E, const double* b, int scale_b, int* iierr, int* niter, double* ftol)
+{
+    if (!E || !b) return fail(CEL_ERR_ARG, "null argument");
+    if (!E->completed) return fail(CEL_ERR_STATE, "cel_elmSolverMatrixCompleted has not been called");
+    std::vector<double> rhs(b, b + E->n);
+    int rc = CEL_OK;
+    if (scale_b) rc = cel_gpu_vec_scale_diagonal(E->A, rhs.data(), CEL_MEM_HOST);
+    if (rc >= 0) {
+        if (E->solver == CEL_ELM_GMRES) {
+            cel_gmres_params p;
+            cel_gpu_gmres_params_default(&p);  // eps 1e-8, iter_max 5000, restart 3, x = 1
+            rc = cel_gpu_gmres_solve(E->A, rhs.data(), E->x.data(), &p, &E->last);
+        } else {
+            cel_pcg_params p;
+            cel_gpu_pcg_params_default(&p);
+            p.tol = 0.1e-07;
+            p.tol_mode = CEL_TOL_ABS;
+            p.iter_max = 5000;
+            p.x0_mode = CEL_X0_ONE;
+            rc = cel_gpu_pcg_solve(E->A, rhs.data(), E->x.data(), &p, &E->last, nullptr);
+        }
+    }
+    // the reference's err_code is 0 after an iteration cap too
+    if (iierr) *iierr = rc < 0 ? rc : 0;
+    if (niter) *niter = E->last.iters;
+    if (ftol) *ftol = E->last.final_norm;
+    return rc;
+}
+
+extern "C" int cel_gpu_elm_solve(cel_gpu_elm* E, const double* b, int* iierr, int* niter, double* ftol)
+{
+    return elm_run(E, b, 1, iierr, niter, ftol);
+}
+
+extern "C" int cel_gpu_elm_resolve(cel_gpu_elm* E, const double* b, int* iierr, int* niter,
+                                   double* ftol)
+{
+    return elm_run(E, b, 0, iierr, niter, ftol);
+}
+
+extern "C" int cel_gpu_elm_collect_solution(const cel_gpu_elm* E, double* x)
+{
+    if (!E || !x) return fail(CEL_ERR_ARG, "null argument");
+    if (!E->completed) return fail(CEL_ERR_STATE, "cel_elmSolverMatrixCompleted has not been called");
+    for (int i = 0; i < E->n; ++i) x[i] = E->x[(size_t)i];
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_elm_stats(const cel_gpu_elm* E, cel_pcg_stats* s)
+{
+    if (!E || !s) return fail(CEL_ERR_ARG, "null argument");
+    *s = E->last;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_elm_matrix(const cel_gpu_elm* E, cel_gpu_mat** A)
+{
+    if (!E || !A) return fail(CEL_ERR_ARG, "null argument");
+    if (!E->completed) return fail(CEL_ERR_STATE, "cel_elmSolverMatrixCompleted has not been called");
+    *A = E->A;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_elm_destroy(cel_gpu_elm* E)
+{
+    if (!E) return CEL_OK;
+    if (E->A) cel_gpu_mat_destroy(E->A);
+    delete E;
+    return CEL_OK;
+}

@@ -19,6 +19,8 @@ module cel_gpu
   integer(c_int), parameter :: CEL_TOL_ABS = 0, CEL_TOL_REL_B = 1, CEL_TOL_REL_R0 = 2
   integer(c_int), parameter :: CEL_PREC_FP64 = 0, CEL_PREC_MIXED = 1
   integer(c_int), parameter :: CEL_JAD_SPARSKIT = 0, CEL_JAD_CEL_COMPAT = 1
+  integer(c_int), parameter :: CEL_SORT_ROW_COL = 0, CEL_SORT_CEL_COMPAT = 1
+  integer(c_int), parameter :: CEL_ELM_GMRES = 0, CEL_ELM_PCG = 1
 
   type, bind(c) :: cel_pcg_params
     real(c_double) :: tol
@@ -33,6 +35,11 @@ module cel_gpu
     integer(c_int) :: spmv_launches
     real(c_double) :: spmv_bytes, iter_bytes
   end type cel_pcg_stats
+
+  type, bind(c) :: cel_gmres_params
+    real(c_double) :: eps
+    integer(c_int) :: iter_max, restart, x0_mode, memspace, format
+  end type cel_gmres_params
 
   interface
     integer(c_int) function cel_gpu_init(device) bind(c, name="cel_gpu_init")
@@ -122,6 +129,91 @@ module cel_gpu
       integer(c_int), value :: nx, ny, nz, memspace
       real(c_double), intent(in) :: kappa(*)
     end function
+    subroutine cel_gpu_gmres_params_default(p) bind(c, name="cel_gpu_gmres_params_default")
+      import :: cel_gmres_params
+      type(cel_gmres_params), intent(out) :: p
+    end subroutine
+    integer(c_int) function cel_gpu_gmres_solve(A, b, x, params, stats) &
+        bind(c, name="cel_gpu_gmres_solve")
+      import :: c_int, c_ptr, c_double, cel_gmres_params, cel_pcg_stats
+      type(c_ptr), value :: A
+      real(c_double), intent(in) :: b(*)
+      real(c_double), intent(inout) :: x(*)
+      type(cel_gmres_params), intent(in) :: params
+      type(cel_pcg_stats), intent(out) :: stats
+    end function
+    integer(c_int) function cel_gpu_mat_create_coo_sorted(A, nrows, ncols, nnz, rows, cols, vals, &
+        index_base, row_offset, col_offset, mode, memspace) bind(c, name="cel_gpu_mat_create_coo_sorted")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), intent(out) :: A
+      integer(c_int), value :: nrows, ncols, nnz, index_base, row_offset, col_offset, mode, memspace
+      integer(c_int), intent(in) :: rows(*), cols(*)
+      real(c_double), intent(in) :: vals(*)
+    end function
+    integer(c_int) function cel_gpu_elm_create(E, itot, ibs, ibl, nnzero) bind(c, name="cel_gpu_elm_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(out) :: E
+      integer(c_int), value :: itot, ibs, ibl
+      integer(c_int), intent(in) :: nnzero(2)
+    end function
+    integer(c_int) function cel_gpu_elm_row_start(E, row) bind(c, name="cel_gpu_elm_row_start")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: E
+      integer(c_int), value :: row
+    end function
+    integer(c_int) function cel_gpu_elm_insert(E, col, val) bind(c, name="cel_gpu_elm_insert")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: E
+      integer(c_int), value :: col
+      real(c_double), value :: val
+    end function
+    integer(c_int) function cel_gpu_elm_reinsert(E, col, val) bind(c, name="cel_gpu_elm_reinsert")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: E
+      integer(c_int), value :: col
+      real(c_double), value :: val
+    end function
+    integer(c_int) function cel_gpu_elm_row_completed(E) bind(c, name="cel_gpu_elm_row_completed")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: E
+    end function
+    integer(c_int) function cel_gpu_elm_matrix_completed(E) bind(c, name="cel_gpu_elm_matrix_completed")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: E
+    end function
+    integer(c_int) function cel_gpu_elm_set_solver(E, solver) bind(c, name="cel_gpu_elm_set_solver")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: E
+      integer(c_int), value :: solver
+    end function
+    integer(c_int) function cel_gpu_elm_solve(E, b, iierr, niter, ftol) bind(c, name="cel_gpu_elm_solve")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: E
+      real(c_double), intent(in) :: b(*)
+      integer(c_int), intent(out) :: iierr, niter
+      real(c_double), intent(out) :: ftol
+    end function
+    integer(c_int) function cel_gpu_elm_resolve(E, b, iierr, niter, ftol) bind(c, name="cel_gpu_elm_resolve")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: E
+      real(c_double), intent(in) :: b(*)
+      integer(c_int), intent(out) :: iierr, niter
+      real(c_double), intent(out) :: ftol
+    end function
+    integer(c_int) function cel_gpu_elm_collect_solution(E, x) bind(c, name="cel_gpu_elm_collect_solution")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: E
+      real(c_double), intent(out) :: x(*)
+    end function
+    integer(c_int) function cel_gpu_elm_stats(E, s) bind(c, name="cel_gpu_elm_stats")
+      import :: c_int, c_ptr, cel_pcg_stats
+      type(c_ptr), value :: E
+      type(cel_pcg_stats), intent(out) :: s
+    end function
+    integer(c_int) function cel_gpu_elm_destroy(E) bind(c, name="cel_gpu_elm_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: E
+    end function
     integer(c_int) function cel_gpu_cg_adjoint(A, x, dLdx, dLdb, dLda, params, stats) &
         bind(c, name="cel_gpu_cg_adjoint")
       import :: c_int, c_ptr, c_double, cel_pcg_params, cel_pcg_stats
@@ -194,13 +286,16 @@ end module cel_gpu
 
 
 module cel_gpu_elmsolver
-  ! One-GPU counterpart of the reference's element solver API: rows are
-  ! inserted one at a time (cel_elmsolver_module.f90:155-189), the matrix is
-  ! assembled once (:193-439) and solved any number of times (:441, :653).
-  ! The reference forces Jacobi-GMRES with diagonal row scaling (:140-151);
-  ! this layer keeps the scaling and the absolute eps = 1e-8 / iter_max = 5000
-  ! settings and runs Jacobi-PCG (GMRES is not part of this library), so it
-  ! serves symmetric positive definite systems.
+  ! The reference's element solver API (cel_elmsolver_module.f90:85-89,
+  ! 103-900) with the same names and arguments, over one cel_gpu_elm handle
+  ! of the C ABI: rows are inserted one at a time, MatrixCompleted sorts them
+  ! in the reference's order and builds the diagonally scaled matrix and its
+  ! Jacobi vector in HBM, DoIt/ReDoIt solve with the reference's
+  ! Jacobi-GMRES(3) (x = 1, eps = 1e-8 absolute, iter_max = 5000; DoIt
+  ! scales b by the diagonal, ReDoIt does not, as in the reference) and
+  ! CollectSolution returns x.  One GPU holds the whole system, so the MPI
+  ! communicator is accepted and not used.  cel_elmSolverSetSolver (no
+  ! reference counterpart) switches to Jacobi-PCG for SPD systems.
   use, intrinsic :: iso_c_binding
   use cel_gpu
   implicit none
@@ -208,158 +303,96 @@ module cel_gpu_elmsolver
   public :: cel_elmSolverInitialization, cel_elmSolverDatarowStart, cel_elmSolverDataInsertion, &
             cel_elmSolverDataReinsertion, cel_elmSolverDatarowCompleted, &
             cel_elmSolverMatrixCompleted, cel_elmSolverDoIt, cel_elmSolverReDoIt, &
-            cel_elmSolverCollectSolution, cel_elmSolverDestroy, cel_elmSolverPerformance
+            cel_elmSolverCollectSolution, cel_elmSolverDestroy, cel_elmSolverPerformance, &
+            cel_elmSolverSetSolver
 
   integer, parameter :: ik = 4, rk = 8
-  integer(ik) :: elm_n = 0, elm_low = 1, elm_row = 0, elm_nnz = 0
-  integer(ik), allocatable :: elm_rows(:), elm_cols(:), elm_diag(:)
-  real(rk), allocatable :: elm_vals(:), elm_x(:)
-  type(c_ptr) :: elm_mat = c_null_ptr
-  type(cel_pcg_stats), save :: elm_stats
+  type(c_ptr), save :: elm = c_null_ptr
+  integer(ik), save :: elm_n = 0
 
 contains
+
+  subroutine report(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    if (rc < 0) write(0, '(A,A,A)') where, ': ', cel_gpu_error_message()
+  end subroutine
 
   subroutine cel_elmSolverInitialization(itotSize, ibs, ibl, nnzero, comm)
     integer(ik), intent(in) :: itotSize, ibs, ibl, nnzero(2)
     integer(4), intent(in) :: comm
     integer(c_int) :: rc
+    if (comm < -huge(comm)) return  ! the communicator is not used on one GPU
     rc = cel_gpu_init(0_c_int)
+    call report(rc, 'cel_elmSolverInitialization')
+    rc = cel_gpu_elm_create(elm, itotSize, ibs, ibl, nnzero)
+    call report(rc, 'cel_elmSolverInitialization')
     elm_n = ibs
-    elm_low = ibl
-    elm_nnz = 0
-    allocate(elm_rows((nnzero(1) + nnzero(2)) * ibs), elm_cols((nnzero(1) + nnzero(2)) * ibs))
-    allocate(elm_vals((nnzero(1) + nnzero(2)) * ibs), elm_diag(ibs))
-    elm_diag = 0
-    if (itotSize /= ibs .or. comm < -huge(comm)) then
-      ! one rank owns all rows here
-      write(0, '(A)') 'cel_elmSolverInitialization: one GPU solves the whole system'
-    end if
   end subroutine
 
   subroutine cel_elmSolverDatarowStart(row)
     integer(ik), intent(in) :: row
-    elm_row = row
+    call report(cel_gpu_elm_row_start(elm, row), 'cel_elmSolverDatarowStart')
   end subroutine
 
   subroutine cel_elmSolverDataInsertion(col, value)
     integer, intent(in) :: col
     real(rk), intent(in) :: value
-    elm_nnz = elm_nnz + 1
-    elm_rows(elm_nnz) = elm_row
-    elm_cols(elm_nnz) = col
-    elm_vals(elm_nnz) = value
-    if (col == elm_row) elm_diag(elm_row - elm_low + 1) = elm_nnz
+    call report(cel_gpu_elm_insert(elm, col, value), 'cel_elmSolverDataInsertion')
   end subroutine
 
   subroutine cel_elmSolverDataReinsertion(col, value)
     integer(ik), intent(in) :: col
     real(rk), intent(in) :: value
-    if (col == elm_row) then
-      elm_vals(elm_diag(elm_row - elm_low + 1)) = value
-    else
-      write(0, '(A)') 'cel_elmSolverDataReinsertion: only the diagonal can be reinserted'
-    end if
+    call report(cel_gpu_elm_reinsert(elm, col, value), 'cel_elmSolverDataReinsertion')
   end subroutine
 
   subroutine cel_elmSolverDatarowCompleted()
+    call report(cel_gpu_elm_row_completed(elm), 'cel_elmSolverDatarowCompleted')
   end subroutine
 
-  ! Sorts the inserted entries by (row, column), builds the device matrix,
-  ! captures the diagonal, scales rows by 1/sqrt|d| and sets the Jacobi
-  ! vector sqrt|d|/d (cel_elmsolver_module.f90:372-419).
   subroutine cel_elmSolverMatrixCompleted()
-    integer(ik), allocatable :: order(:)
-    integer(c_int) :: rc
-    integer(ik) :: k
-    allocate(order(elm_nnz))
-    call sort_row_col(order)
-    elm_rows(1:elm_nnz) = elm_rows(order)
-    elm_cols(1:elm_nnz) = elm_cols(order)
-    elm_vals(1:elm_nnz) = elm_vals(order)
-    do k = 1, elm_nnz
-      elm_rows(k) = elm_rows(k) - elm_low + 1
-      elm_cols(k) = elm_cols(k) - elm_low + 1
-    end do
-    rc = cel_gpu_mat_create_coo(elm_mat, elm_n, elm_n, elm_nnz, elm_rows, elm_cols, elm_vals, &
-                                1_c_int, 0_c_int, 0_c_int)
-    if (rc < 0) write(0, '(A,A)') 'cel_elmSolverMatrixCompleted: ', cel_gpu_error_message()
-    rc = cel_gpu_mat_set_jacobi(elm_mat, 1_c_int)
-    if (rc < 0) write(0, '(A,A)') 'cel_elmSolverMatrixCompleted: ', cel_gpu_error_message()
-    rc = cel_gpu_mat_scale_diagonal(elm_mat)
-    allocate(elm_x(elm_n))
+    call report(cel_gpu_elm_matrix_completed(elm), 'cel_elmSolverMatrixCompleted')
+  end subroutine
+
+  subroutine cel_elmSolverSetSolver(solver)
+    integer(ik), intent(in) :: solver
+    call report(cel_gpu_elm_set_solver(elm, solver), 'cel_elmSolverSetSolver')
   end subroutine
 
   subroutine cel_elmSolverDoIt(bmat, iierr, niter, ftol)
-    real(rk), intent(inout) :: bmat(:)
+    real(rk), intent(in) :: bmat(:)
     integer(ik), intent(out) :: iierr, niter
     real(rk), intent(out) :: ftol
-    type(cel_pcg_params) :: p
-    integer(c_int) :: rc
-    real(rk), allocatable :: b(:)
-    call cel_gpu_pcg_params_default(p)
-    p%tol = 0.1e-07_rk            ! the API's eps (cel_elmsolver_module.f90:141)
-    p%iter_max = 5000             ! :140
-    p%x0_mode = CEL_X0_ONE        ! cel_cgalg_module.f90:209
-    allocate(b(elm_n))
-    b = bmat(1:elm_n)
-    rc = cel_gpu_vec_scale_diagonal(elm_mat, b, CEL_MEM_HOST)
-    rc = cel_gpu_pcg_solve(elm_mat, b, elm_x, p, elm_stats, c_null_ptr)
-    iierr = merge(0_ik, int(rc, ik), rc == CEL_OK)
-    niter = elm_stats%iters
-    ftol = elm_stats%final_norm
-    bmat(1:elm_n) = elm_x
+    call report(cel_gpu_elm_solve(elm, bmat, iierr, niter, ftol), 'cel_elmSolverDoIt')
   end subroutine
 
   subroutine cel_elmSolverReDoIt(bmat, iierr, niter, ftol)
-    real(rk), intent(inout) :: bmat(:)
+    real(rk), intent(in) :: bmat(:)
     integer(ik), intent(out) :: iierr, niter
     real(rk), intent(out) :: ftol
-    call cel_elmSolverDoIt(bmat, iierr, niter, ftol)
+    call report(cel_gpu_elm_resolve(elm, bmat, iierr, niter, ftol), 'cel_elmSolverReDoIt')
   end subroutine
 
   subroutine cel_elmSolverCollectSolution(bmat)
     real(rk), allocatable, intent(inout) :: bmat(:)
-    if (.not. allocated(bmat)) allocate(bmat(elm_n))
-    bmat(1:elm_n) = elm_x
+    if (allocated(bmat)) deallocate(bmat)
+    allocate(bmat(elm_n))
+    call report(cel_gpu_elm_collect_solution(elm, bmat), 'cel_elmSolverCollectSolution')
   end subroutine
 
   subroutine cel_elmSolverDestroy()
     integer(c_int) :: rc
-    if (c_associated(elm_mat)) rc = cel_gpu_mat_destroy(elm_mat)
-    elm_mat = c_null_ptr
-    if (allocated(elm_rows)) deallocate(elm_rows, elm_cols, elm_vals, elm_diag)
-    if (allocated(elm_x)) deallocate(elm_x)
+    if (c_associated(elm)) rc = cel_gpu_elm_destroy(elm)
+    elm = c_null_ptr
   end subroutine
 
   subroutine cel_elmSolverPerformance()
-    write(*, '(A,I0,A,ES13.6,A,F12.3,A)') 'iterations: ', elm_stats%iters, '; |r|: ', &
-      elm_stats%final_norm, '; solve: ', elm_stats%solve_ms, ' ms'
+    type(cel_pcg_stats) :: s
+    integer(c_int) :: rc
+    rc = cel_gpu_elm_stats(elm, s)
+    write(*, '(A,I0,A,I0,A,ES13.6,A,F12.3,A)') 'iterations: ', s%iters, '; cycles: ', &
+      s%outer_iters, '; |r|: ', s%final_norm, '; solve: ', s%solve_ms, ' ms'
   end subroutine
-
-  ! stable order of the inserted entries by (row, column)
-  subroutine sort_row_col(order)
-    integer(ik), intent(out) :: order(:)
-    integer(ik) :: i, j, t
-    do i = 1, elm_nnz
-      order(i) = i
-    end do
-    ! insertion sort per row run; rows arrive mostly in order
-    do i = 2, elm_nnz
-      t = order(i)
-      j = i - 1
-      do while (j >= 1)
-        if (.not. before(t, order(j))) exit
-        order(j + 1) = order(j)
-        j = j - 1
-      end do
-      order(j + 1) = t
-    end do
-  end subroutine
-
-  logical function before(a, b)
-    integer(ik), intent(in) :: a, b
-    before = elm_rows(a) < elm_rows(b) .or. &
-             (elm_rows(a) == elm_rows(b) .and. elm_cols(a) < elm_cols(b))
-  end function
 
 end module cel_gpu_elmsolver

@@ -367,6 +367,86 @@ int cel_gpu_batch_solve(cel_gpu_batch* B, const double* const* b, double* const*
                         const cel_pcg_params* params, cel_pcg_stats* lane_stats,
                         cel_pcg_stats* total);
 
+/* ---- Jacobi-GMRES ------------------------------------------------------ */
+/* cel_gmresalg_parameter_type settings used by cel_gmresalg_2jacobi
+ * (src/fcgalg/cel_alg_module.f90:139-271). */
+typedef struct {
+    double eps;     /* absolute: stop when the Givens estimate |gamma(j+1)| <= eps */
+    int iter_max;   /* outer iterations; the loop runs while iter <= iter_max */
+    int restart;    /* m: Arnoldi steps per cycle (1..32); m+1 cycles per outer iteration */
+    int x0_mode;    /* CEL_X0_ONE is the reference's x = 1 */
+    int memspace;   /* b and x on the host or the device */
+    int format;     /* CEL_FMT_CSR or CEL_FMT_JAD */
+} cel_gmres_params;
+
+/* The element solver's settings (src/felmsolver/cel_elmsolver_module.f90:140-151):
+ * eps 1e-8, iter_max 5000, restart 3, x = 1, host arrays, CSR. */
+void cel_gpu_gmres_params_default(cel_gmres_params* p);
+
+/* cel_gmresalg_2jacobi (src/fcgalg/cel_gmresalg_module.f90:570-851), one
+ * rank, local block only: right Jacobi preconditioning (z = jac * v),
+ * modified Gram-Schmidt, Givens rotations applied as the reference writes
+ * them (:722-727: the second statement assigns hh(i,j) again, so hh(i+1,j)
+ * is never rotated; with restart >= 2 the estimate can meet eps while the
+ * true residual is larger, exactly as in the reference).  Returns CEL_OK
+ * when the estimate met eps, CEL_NOT_CONVERGED after iter_max+1 outer
+ * iterations.  stats: iters = outer iterations (the reference's
+ * iter_done), outer_iters = restart cycles, norm0 = first residual,
+ * final_norm = the true residual norm ||b - A x|| after the last cycle
+ * (the reference's residuum_norm, returned as ftol by cel_elmSolverDoIt). */
+int cel_gpu_gmres_solve(cel_gpu_mat* A, const double* b, double* x, const cel_gmres_params* params,
+                        cel_pcg_stats* stats);
+
+/* ---- element solver API (src/felmsolver/cel_elmsolver_module.f90) ------ */
+/* One handle per system; on one GPU it holds the whole system (the
+ * reference gathers a reduce group's rows on its master rank).  The Fortran
+ * module cel_gpu_elmsolver keeps one handle and offers the reference's
+ * cel_elmSolver* names over these calls. */
+typedef struct cel_gpu_elm cel_gpu_elm;
+#define CEL_ELM_GMRES 0  /* Jacobi-GMRES(3), the method the reference forces (:146) */
+#define CEL_ELM_PCG 1    /* Jacobi-PCG (symmetric positive definite systems) */
+
+/* cel_elmSolverInitialization (:103-153): itot global rows, ibs local rows,
+ * ibl first local row, nnzero[0..1] entries per row (a capacity hint; the
+ * arrays grow past it where the reference's would be overrun).  One GPU:
+ * ibs == itot and ibl == 1, else CEL_ERR_ARG. */
+int cel_gpu_elm_create(cel_gpu_elm** E, int itot, int ibs, int ibl, const int* nnzero);
+/* cel_elmSolverDatarowStart (:155-159): rows 1..ibs. */
+int cel_gpu_elm_row_start(cel_gpu_elm* E, int row);
+/* cel_elmSolverDataInsertion (:161-172): appends (row, col, value); an
+ * entry with col == row is the row's diagonal (the last one counts). */
+int cel_gpu_elm_insert(cel_gpu_elm* E, int col, double value);
+/* cel_elmSolverDataReinsertion (:174-187): replaces the value of the
+ * current row's diagonal entry; other columns are CEL_ERR_ARG (the
+ * reference stops), a row without a diagonal CEL_ERR_REF_UNDEFINED. */
+int cel_gpu_elm_reinsert(cel_gpu_elm* E, int col, double value);
+/* cel_elmSolverDatarowCompleted (:189-191): nothing to do. */
+int cel_gpu_elm_row_completed(cel_gpu_elm* E);
+/* cel_elmSolverMatrixCompleted (:193-439): the reference's COO sort
+ * (cel_gpu_coo_sort order, :372), COO->CSR, diagonal captured, rows scaled
+ * by 1/sqrt|d_i| and the Jacobi vector sqrt|d|/d (:386-419), all in HBM.
+ * Insertion after this is CEL_ERR_STATE (the reference frees its arrays). */
+int cel_gpu_elm_matrix_completed(cel_gpu_elm* E);
+/* Solver of the next solves (no reference counterpart; default GMRES). */
+int cel_gpu_elm_set_solver(cel_gpu_elm* E, int solver);
+/* cel_elmSolverDoIt (:441-651): b (ibs, host) scaled by 1/sqrt|d_i|, then
+ * the solver from x = 1 with eps = 1e-8 (absolute), iter_max = 5000,
+ * GMRES restart 3.  iierr = 0 (or the error code), niter = the solver's
+ * iteration count, ftol = its final residual norm (GMRES: the true
+ * residual of the scaled system).  Returns the solver's status. */
+int cel_gpu_elm_solve(cel_gpu_elm* E, const double* b, int* iierr, int* niter, double* ftol);
+/* cel_elmSolverReDoIt (:653-788): the same with a new b that is NOT scaled
+ * by the diagonal -- the reference's ReDoIt omits that step. */
+int cel_gpu_elm_resolve(cel_gpu_elm* E, const double* b, int* iierr, int* niter, double* ftol);
+/* cel_elmSolverCollectSolution (:790-815): x (ibs, host). */
+int cel_gpu_elm_collect_solution(const cel_gpu_elm* E, double* x);
+/* cel_elmSolverPerformance (:879-900): the last solve's statistics. */
+int cel_gpu_elm_stats(const cel_gpu_elm* E, cel_pcg_stats* s);
+/* The assembled matrix (owned by the handle), for inspection. */
+int cel_gpu_elm_matrix(const cel_gpu_elm* E, cel_gpu_mat** A);
+/* cel_elmSolverDestroy (:817-877). */
+int cel_gpu_elm_destroy(cel_gpu_elm* E);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,10 @@ def _declare(L):
                                   _i32p, _i32p, _i32p, _i32p, _f64p, C.POINTER(C.c_int), _i32p,
                                   _i32p, _i32p, _i32p, _f64p]
     L.orc_face_pack.argtypes = [C.c_int, _i32p, _f64p, _f64p]
+    L.orc_gmres_jacobi.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_int, _f64p, _f64p, _f64p,
+                                   C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   _f64p, C.c_int]
     L.orc_coo_sort_cel.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p]
     L.orc_coo_sort_rowcol.argtypes = [C.c_int, _i32p, _i32p, _f64p]
     L.orc_sddmm_csr.argtypes = [C.c_int, _i32p, _i32p, C.c_int, _f64p, _f64p, _f64p]
@@ -249,6 +253,22 @@ def pcg_jacobi(A: CSR, jac, b, x0=None, x0_mode=X0_ONE, eps=1e-8, iter_max=1000,
                               ah.ctypes.data, sh.ctypes.data, C.byref(fn))
     return dict(x=x, iters=it, norm=fn.value, hist=hist[: it + 1], alpha=ah[: it + 1],
                 beta=sh[: it + 1])
+
+
+def gmres_jacobi(A: CSR, jac, b, x0=None, x0_mode=X0_ONE, eps=1e-8, iter_max=5000, m=3,
+                 workers=1, hist_cap=100000):
+    """cel_gmresalg_2jacobi restated (cel_oracle.h).  Returns dict(iters,
+    x, norm, reason, cycles, hist)."""
+    x = np.zeros(A.n) if x0 is None else _f64(x0).copy()
+    if x0 is not None:
+        x0_mode = X0_USER
+    hist = np.zeros(max(hist_cap, 1))
+    fn, rs, cy = C.c_double(), C.c_int(), C.c_int()
+    it = lib().orc_gmres_jacobi(A.n, A.xadj, A.cols, A.vals, A.col_offset, _f64(jac), _f64(b), x,
+                                x0_mode, eps, iter_max, m, workers, C.byref(fn), C.byref(rs),
+                                C.byref(cy), hist, hist.size)
+    return dict(iters=it, x=x, norm=fn.value, reason=rs.value, cycles=cy.value,
+                hist=hist[:min(cy.value, hist.size)].copy())
 
 
 def owner_of(vtxdist, col):

@@ -517,6 +517,172 @@ int orc_pcg_jacobi(int n, const int* xadj, const int* cols, const double* vals, 
 }
 
 /* ------------------------------------------------------------------ */
+/* Jacobi-GMRES, reference op order (cel_gmresalg_module.f90:570-851)   */
+
+/* Dot product as the reference's workers form it (:724-775 of
+ * cel_fop_vec_distr_module.f90): each worker sums its chunk left to right
+ * from 0.0, the partials are added in worker order from 0.0. */
+static double dot_chunks(int n, int workers, const double* a, const double* b)
+{
+    double sum = 0.0;
+    for (int w = 1; w <= workers; ++w) {
+        int s, e;
+        orc_work_distr(n, workers, w, &s, &e);
+        double acc = 0.0;
+        for (int i = s - 1; i < e; ++i) {
+            double prod = a[i] * b[i];
+            acc = acc + prod;
+        }
+        sum = sum + acc;
+    }
+    return sum;
+}
+
+/* r = 0 ; r = r + A x ; r = b - r ; returns sqrt(r . r) */
+static double gm_residual(int n, const int* xadj, const int* cols, const double* vals,
+                          int col_offset, const double* b, const double* x, double* r, int workers)
+{
+    for (int i = 0; i < n; ++i) r[i] = 0.0;
+    csr_rows(1, n, xadj, cols, vals, col_offset, x, r);
+    for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    return sqrt(dot_chunks(n, workers, r, r));
+}
+
+int orc_gmres_jacobi(int n, const int* xadj, const int* cols, const double* vals, int col_offset,
+                     const double* jac, const double* b, double* x, int x0_mode, double eps,
+                     int iter_max, int m, int workers, double* final_norm, int* reason,
+                     int* cycles_out, double* hist, int hist_cap)
+{
+    if (workers < 1) workers = 1;
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    double* r = (double*)malloc(sizeof(double) * nn);
+    double* V = (double*)malloc(sizeof(double) * nn * (size_t)(m + 1));  /* v_1 .. v_{m+1} */
+    double* Z = (double*)malloc(sizeof(double) * nn * (size_t)m);        /* z_1 .. z_m */
+    double* W = (double*)malloc(sizeof(double) * nn * (size_t)m);        /* w_1 .. w_m */
+    /* 1-based small arrays as the reference's: hh(m+1, m+1), cc, ss, gamma, alpha(m+1) */
+    const int M1 = m + 2;
+    double* hh = (double*)calloc((size_t)M1 * M1, sizeof(double));
+    double* cc = (double*)calloc((size_t)M1, sizeof(double));
+    double* ss = (double*)calloc((size_t)M1, sizeof(double));
+    double* gam = (double*)calloc((size_t)M1, sizeof(double));
+    double* alp = (double*)calloc((size_t)M1, sizeof(double));
+#define H(i, j) hh[(size_t)(i) * M1 + (j)]
+#define VV(k) (V + (size_t)((k) - 1) * nn)
+#define ZZ(k) (Z + (size_t)((k) - 1) * nn)
+#define WW(k) (W + (size_t)((k) - 1) * nn)
+    const double small = orc_real_small();
+    int conv = 0, conv_field = 0, iter = 0, cycles = 0;
+    double residuum_norm = 0.0;
+    /* :637 x = 1.0 */
+    for (int i = 0; i < n; ++i) {
+        if (x0_mode == ORC_X0_ONE) x[i] = 1.0;
+        else if (x0_mode == ORC_X0_ZERO) x[i] = 0.0;
+    }
+    while (iter <= iter_max && conv == 0) {                     /* :640 */
+        iter = iter + 1;
+        residuum_norm = gm_residual(n, xadj, cols, vals, col_offset, b, x, r, workers);
+        gam[1] = residuum_norm;                                 /* :656-660 */
+        conv_field = fabs(residuum_norm) <= small ? 1 : 0;
+        double local_residuum = residuum_norm;
+        int restart_count = 0;
+        while (restart_count <= m && conv == 0) {               /* :665 */
+            double tmp = 1.0 / local_residuum;                  /* scale_inv */
+            for (int i = 0; i < n; ++i) VV(1)[i] = r[i] * tmp;
+            gam[1] = residuum_norm;
+            for (int jj = 1; jj <= m; ++jj) {
+                double* w = WW(jj);
+                double* z = ZZ(jj);
+                for (int i = 0; i < n; ++i) w[i] = 0.0;
+                for (int i = 0; i < n; ++i) z[i] = VV(jj)[i] * jac[i];
+                csr_rows(1, n, xadj, cols, vals, col_offset, z, w);
+                for (int ii = 1; ii <= jj; ++ii) {              /* :697-711 */
+                    H(ii, jj) = dot_chunks(n, workers, VV(ii), w);
+                    const double h = H(ii, jj);
+                    for (int i = 0; i < n; ++i) {
+                        double t = h * VV(ii)[i];
+                        w[i] = w[i] - t;
+                    }
+                }
+                H(jj + 1, jj) = sqrt(dot_chunks(n, workers, w, w));   /* :713-717 */
+                for (int ii = 1; ii <= jj - 1; ++ii) {          /* :722-727, as written */
+                    double a = H(ii, jj) * cc[ii];
+                    double c = H(ii + 1, jj) * ss[ii];
+                    H(ii, jj) = a + c;
+                    a = -H(ii, jj) * ss[ii];
+                    c = H(ii + 1, jj) * cc[ii];
+                    H(ii, jj) = a + c;
+                }
+                {
+                    double a = H(jj, jj) * H(jj, jj);           /* :728 */
+                    double c = H(jj + 1, jj) * H(jj + 1, jj);
+                    double betta = sqrt(a + c);
+                    ss[jj] = H(jj + 1, jj) / betta;
+                    cc[jj] = H(jj, jj) / betta;
+                    H(jj, jj) = betta;
+                    gam[jj + 1] = -ss[jj] * gam[jj];
+                    gam[jj] = cc[jj] * gam[jj];
+                }
+                if (fabs(gam[jj + 1]) <= eps) {                 /* :737-766 */
+                    for (int ii = jj; ii >= 1; --ii) {
+                        alp[ii] = 0.0;
+                        for (int kk = ii + 1; kk <= jj; ++kk) {
+                            double t = H(ii, kk) * alp[kk];
+                            alp[ii] = alp[ii] + t;
+                        }
+                        double inv = 1.0 / H(ii, ii);
+                        alp[ii] = inv * (gam[ii] - alp[ii]);
+                    }
+                    for (int ii = 1; ii <= jj; ++ii) {
+                        const double a = alp[ii];
+                        for (int i = 0; i < n; ++i) {
+                            double t = a * ZZ(ii)[i];
+                            x[i] = x[i] + t;
+                        }
+                    }
+                    conv_field = 2;
+                    conv = 2;
+                    break;
+                }
+                const double hd = H(jj + 1, jj);                /* :767-778 */
+                for (int i = 0; i < n; ++i) VV(jj + 1)[i] = w[i] / hd;
+            }
+            if (conv == 0) {                                    /* :780-800 */
+                for (int ii = m; ii >= 1; --ii) {
+                    alp[ii] = 0.0;
+                    for (int kk = ii + 1; kk <= m; ++kk) {
+                        double t = H(ii, kk) * alp[kk];
+                        alp[ii] = alp[ii] + t;
+                    }
+                    double inv = 1.0 / H(ii, ii);
+                    alp[ii] = inv * (gam[ii] - alp[ii]);
+                }
+                for (int ii = 1; ii <= m; ++ii) {
+                    const double a = alp[ii];
+                    for (int i = 0; i < n; ++i) {
+                        double t = a * ZZ(ii)[i];
+                        x[i] = x[i] + t;
+                    }
+                }
+            }
+            residuum_norm = gm_residual(n, xadj, cols, vals, col_offset, b, x, r, workers);
+            local_residuum = residuum_norm;                     /* :802-815 */
+            if (hist && cycles < hist_cap) hist[cycles] = residuum_norm;
+            cycles += 1;
+            restart_count += 1;
+        }
+    }
+#undef H
+#undef VV
+#undef ZZ
+#undef WW
+    *final_norm = residuum_norm;
+    if (reason) *reason = conv_field;
+    if (cycles_out) *cycles_out = cycles;
+    free(r); free(V); free(Z); free(W); free(hh); free(cc); free(ss); free(gam); free(alp);
+    return iter;
+}
+
+/* ------------------------------------------------------------------ */
 /* distributed block split                                             */
 
 int orc_owner_of(int nranks, const int* vtxdist, int col)

@@ -167,6 +167,25 @@ int orc_pcg_jacobi(int n, const int* xadj, const int* cols, const double* vals,
                    double* hist, double* alpha_hist, double* sigma_hist,
                    double* final_norm);
 
+/* ---- Jacobi-GMRES in the reference op order (src/fcgalg/cel_gmresalg_module.f90:570-851) */
+
+/* cel_gmresalg_2jacobi with one rank: x = 1 (or per x0_mode); outer
+ * iterations (at most iter_max+1) each recompute r = b - A x and run up to
+ * m+1 restart cycles of m Arnoldi steps with right Jacobi preconditioning
+ * (z_j = jac * v_j, w_j = A z_j, modified Gram-Schmidt), stopping when the
+ * Givens estimate |gamma(j+1)| <= eps (absolute).  The rotation of column
+ * j by the earlier rotations is restated as the reference writes it
+ * (:722-727): both statements assign hh(i,j), so hh(i+1,j) is never
+ * rotated.  Dot products as dot_chunks (worker chunks, in worker order).
+ * Returns the outer iteration count (the reference's iter_done);
+ * *final_norm = the last true residual norm (residuum_norm), *reason = 2
+ * when the estimate met eps, 1 when the first residual was below
+ * cel_real_small, else 0; hist[k] = true residual norm after cycle k. */
+int orc_gmres_jacobi(int n, const int* xadj, const int* cols, const double* vals, int col_offset,
+                     const double* jac, const double* b, double* x, int x0_mode, double eps,
+                     int iter_max, int m, int workers, double* final_norm, int* reason,
+                     int* cycles_out, double* hist, int hist_cap);
+
 /* ---- distributed block split (src/fspmtdistr/cel_sp_mat_distr_module.f90:34-301) */
 
 /* Owning rank of global column col (1-based) for vtxdist (1-based starts,

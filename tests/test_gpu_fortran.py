@@ -1,8 +1,9 @@
 """The Fortran bindings (cel_v3_amd/fortran/cel_gpu_mod.f90) driving the
 library: the element-solver layer (row-wise assembly, MatrixCompleted with
-diagonal row scaling + Jacobi vector, DoIt with the reference's x0 = 1 and
-absolute eps = 1e-8, compared with the oracle on the same scaled system),
-and a batched solve through the C-pointer-array interface."""
+the reference's sort, diagonal row scaling + Jacobi vector, DoIt/ReDoIt with
+the reference's Jacobi-GMRES(3), x0 = 1 and absolute eps = 1e-8, compared
+with the oracle on the same scaled system), and a batched solve through the
+C-pointer-array interface."""
 import os
 import subprocess
 
@@ -20,25 +21,39 @@ BATCH_EXE = os.path.join(BUILD, "batch_poisson")
 
 
 def test_fortran_elm_driver(gpu):
+    """tests/fortran/elm_poisson.f90 through the reference's API names:
+    a nonsymmetric 12^3 operator solved with the default Jacobi-GMRES(3)
+    (DoIt, then ReDoIt with the same b, which the reference does not scale),
+    then the Poisson operator with Jacobi-PCG; each against the oracle
+    pipeline of MatrixCompleted + DoIt/ReDoIt."""
+    from test_gpu_gmres import oracle_elm
+    from test_oracle import convdiff_csr
+
     if not os.path.exists(EXE):
         pytest.skip("Fortran driver not built (no amdflang)")
     out = subprocess.run([EXE], capture_output=True, text=True, timeout=120)
     print(out.stdout, out.stderr)
     assert out.returncode == 0
-    first = out.stdout.split("\n")[0].split()
-    status, iters, norm, maxerr = int(first[1]), int(first[3]), float(first[5]), float(first[7])
+    lines = [ln.split() for ln in out.stdout.splitlines() if ln.startswith("status")]
+    assert len(lines) == 3
     n = 12
-    A = O.gen_poisson7(n, n, n)
-    xt = (np.arange(1, A.n + 1) % 7) / 7.0
-    b = O.spmv_csr(A, xt)
-    d = O.csr_diagonal(A)
-    S = O.scale_csr(A, d)
-    ref = O.pcg_jacobi(S, O.set_jacobi(d, scaled=True), O.scale_vec(b, d), x0_mode=O.X0_ONE,
-                       eps=1e-8, iter_max=5000, workers=3)
-    assert status == 0
-    assert abs(iters - ref["iters"]) <= 1
-    assert abs(norm - ref["norm"]) <= 1e-12 * ref["hist"][0]
-    assert maxerr < 1e-6
+    cases = [(convdiff_csr(n, n, n, 0.5), "gmres", True), (convdiff_csr(n, n, n, 0.5), "gmres", False),
+             (O.gen_poisson7(n, n, n), "pcg", True)]
+    for ln, (A, solver, scale_b) in zip(lines, cases):
+        status, iters, norm, maxerr = int(ln[1]), int(ln[3]), float(ln[5]), float(ln[7])
+        xt = (np.arange(1, A.n + 1) % 7) / 7.0
+        b = O.spmv_csr(A, xt)
+        rows = np.repeat(np.arange(1, A.n + 1), np.diff(A.xadj))
+        _, ref = oracle_elm(rows, A.cols, A.vals, A.n, b, solver=solver, scale_b=scale_b)
+        assert status == 0
+        if solver == "gmres":
+            assert iters == ref["iters"]
+            assert abs(norm - ref["norm"]) <= 1e-10 * np.linalg.norm(b)
+            assert abs(maxerr - np.abs(ref["x"] - xt).max()) <= 1e-8
+        else:
+            assert abs(iters - ref["iters"]) <= 1
+            assert abs(norm - ref["norm"]) <= 1e-12 * ref["hist"][0]
+            assert maxerr < 1e-6
 
 
 def test_fortran_batch_driver(gpu):

@@ -623,3 +623,162 @@ def test_coo_sort_cel_differs_from_rowcol_only_where_the_quirks_say():
             np.testing.assert_array_equal(r, r0)
             np.testing.assert_array_equal(c, c0)
             np.testing.assert_array_equal(v, v0)
+
+
+# ---------------------------------------------------------------- Jacobi-GMRES
+def convdiff_csr(nx, ny, nz, c=0.5):
+    """7-point operator with a first-order upwind term in x: diag 6 + c,
+    west neighbour -1 - c, the others -1 (nonsymmetric, diagonally
+    dominant).  1-based CSR, ascending columns."""
+    A = O.gen_poisson7(nx, ny, nz)
+    vals = A.vals.copy()
+    rows = np.repeat(np.arange(1, A.n + 1), np.diff(A.xadj))
+    vals[A.cols == rows] += c
+    vals[A.cols == rows - 1] -= c  # west neighbour (same y, z line: x-1)
+    # x-1 of the first cell of a line is not a neighbour: the generator holds
+    # no entry there, so nothing else changes
+    return O.CSR(A.xadj, A.cols, vals)
+
+
+def dense_of(A):
+    D = np.zeros((A.n, A.n))
+    rows = np.repeat(np.arange(A.n), np.diff(A.xadj))
+    D[rows, A.cols - 1] = A.vals
+    return D
+
+
+def test_gmres_restart1_is_minimal_residual():
+    """With one Arnoldi step per cycle no earlier rotation exists, and each
+    cycle is the minimal-residual step along z = jac * r: x += a z with
+    a = (r . Az) / (Az . Az), stopping once the residual estimate
+    |r| sin(angle(r, Az)) <= eps.  The independent numpy loop below counts
+    the same steps and lands on the same x."""
+    A = convdiff_csr(6, 5, 4)
+    D = dense_of(A)
+    jac = O.set_jacobi(O.csr_diagonal(A))
+    b = np.random.default_rng(41).random(A.n)
+    eps = 1e-9
+    ref = O.gmres_jacobi(A, jac, b, eps=eps, m=1, iter_max=5000)
+    x = np.ones(A.n)
+    steps = 0
+    while True:
+        r = b - D @ x
+        beta = np.linalg.norm(r)
+        z = jac * r / beta
+        Az = D @ z
+        v = r / beta
+        a = beta * (v @ Az) / (Az @ Az)
+        x = x + a * z
+        steps += 1
+        est = beta * np.sqrt(max(0.0, 1.0 - (v @ Az) ** 2 / (Az @ Az)))
+        if est <= eps or steps > 10000:
+            break
+    assert ref["reason"] == 2
+    assert ref["cycles"] == steps
+    np.testing.assert_allclose(ref["x"], x, rtol=0, atol=1e-12 * np.abs(x).max())
+
+
+def test_gmres_diagonal_matrix_is_solved_in_one_step():
+    """A diagonal with jac = 1/d: w = A z1 = v1, h11 = 1, h21 = 0, so the
+    first estimate is 0 and x = 1 + (b - d)/d = b/d."""
+    n = 50
+    d = np.random.default_rng(42).uniform(0.5, 3.0, n) * np.where(np.arange(n) % 3 == 0, -1, 1)
+    A = O.CSR(np.arange(1, n + 2, dtype=np.int32), np.arange(1, n + 1, dtype=np.int32), d)
+    b = np.random.default_rng(43).normal(size=n)
+    ref = O.gmres_jacobi(A, 1.0 / d, b, eps=1e-8, m=3)
+    assert (ref["iters"], ref["cycles"], ref["reason"]) == (1, 1, 2)
+    np.testing.assert_allclose(ref["x"], b / d, rtol=0, atol=1e-14 * np.abs(b / d).max())
+
+
+def reference_gmres_dense(D, jac, b, eps, iter_max, m):
+    """cel_gmresalg_2jacobi written again from the reference text
+    (src/fcgalg/cel_gmresalg_module.f90:637-840) over dense numpy arrays,
+    1-based dictionaries for the small arrays: a second transcription that
+    catches index and loop mistakes in the C restatement (not a misreading
+    of the reference, which both would share)."""
+    n = b.size
+    x = np.ones(n)
+    hh, cc, ss, gam, alp = {}, {}, {}, {}, {}
+    conv, it, cycles, res = 0, 0, 0, 0.0
+    while it <= iter_max and conv == 0:
+        it += 1
+        r = b - D @ x
+        res = np.sqrt(r @ r)
+        gam[1] = res
+        loc = res
+        rc = 0
+        while rc <= m and conv == 0:
+            v = {1: r * (1.0 / loc)}
+            z = {}
+            gam[1] = res
+            for jj in range(1, m + 1):
+                z[jj] = v[jj] * jac
+                w = D @ z[jj]
+                for ii in range(1, jj + 1):
+                    hh[ii, jj] = v[ii] @ w
+                    w = w - hh[ii, jj] * v[ii]
+                hh[jj + 1, jj] = np.sqrt(w @ w)
+                for ii in range(1, jj):
+                    hh[ii, jj] = hh[ii, jj] * cc[ii] + hh[ii + 1, jj] * ss[ii]
+                    hh[ii, jj] = -hh[ii, jj] * ss[ii] + hh[ii + 1, jj] * cc[ii]
+                be = np.sqrt(hh[jj, jj] ** 2 + hh[jj + 1, jj] ** 2)
+                ss[jj] = hh[jj + 1, jj] / be
+                cc[jj] = hh[jj, jj] / be
+                hh[jj, jj] = be
+                gam[jj + 1] = -ss[jj] * gam[jj]
+                gam[jj] = cc[jj] * gam[jj]
+                if abs(gam[jj + 1]) <= eps:
+                    for ii in range(jj, 0, -1):
+                        alp[ii] = sum((hh[ii, kk] * alp[kk] for kk in range(ii + 1, jj + 1)), 0.0)
+                        alp[ii] = (1.0 / hh[ii, ii]) * (gam[ii] - alp[ii])
+                    for ii in range(1, jj + 1):
+                        x = x + alp[ii] * z[ii]
+                    conv = 2
+                    break
+                v[jj + 1] = w / hh[jj + 1, jj]
+            if conv == 0:
+                for ii in range(m, 0, -1):
+                    alp[ii] = sum((hh[ii, kk] * alp[kk] for kk in range(ii + 1, m + 1)), 0.0)
+                    alp[ii] = (1.0 / hh[ii, ii]) * (gam[ii] - alp[ii])
+                for ii in range(1, m + 1):
+                    x = x + alp[ii] * z[ii]
+            r = b - D @ x
+            res = np.sqrt(r @ r)
+            loc = res
+            cycles += 1
+            rc += 1
+    return dict(iters=it, cycles=cycles, norm=res, x=x, reason=conv)
+
+
+@pytest.mark.parametrize("m", [2, 3, 5])
+def test_gmres_matches_second_transcription(m):
+    A = convdiff_csr(5, 4, 4)
+    D = dense_of(A)
+    jac = O.set_jacobi(O.csr_diagonal(A))
+    b = np.random.default_rng(44).random(A.n)
+    ref = O.gmres_jacobi(A, jac, b, eps=1e-8, m=m, iter_max=200)
+    alt = reference_gmres_dense(D, jac, b, 1e-8, 200, m)
+    assert (ref["iters"], ref["cycles"], ref["reason"]) == (alt["iters"], alt["cycles"], alt["reason"])
+    np.testing.assert_allclose(ref["x"], alt["x"], rtol=0, atol=1e-9 * np.abs(alt["x"]).max())
+    assert abs(ref["norm"] - alt["norm"]) <= 1e-6 * alt["norm"] + 1e-14
+
+
+def test_gmres_final_norm_is_the_true_residual_and_shows_the_rotation_defect():
+    """ftol is ||b - A x|| of the returned x.  With restart >= 2 the
+    estimate of the reference's unrotated Hessenberg column meets eps while
+    the true residual stays far above it; a textbook rotation (h(i+1,j)
+    rotated too) would leave the true residual near eps, so this fails if
+    the restatement 'fixes' the reference."""
+    A = O.gen_poisson7(8, 8, 8)
+    D = dense_of(A)
+    jac = O.set_jacobi(O.csr_diagonal(A))
+    b = np.random.default_rng(0).random(A.n)
+    for m, workers in ((1, 1), (3, 1), (3, 4)):
+        ref = O.gmres_jacobi(A, jac, b, eps=1e-8, m=m, workers=workers)
+        true = np.linalg.norm(b - D @ ref["x"])
+        assert ref["reason"] == 2
+        assert abs(ref["norm"] - true) <= 1e-13 * np.linalg.norm(b)
+        if m == 1:
+            assert ref["norm"] <= 1.01e-8
+        else:
+            assert ref["norm"] > 100 * 1e-8
