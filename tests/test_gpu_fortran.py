@@ -49,7 +49,8 @@ def test_fortran_elm_driver(gpu):
         if solver == "gmres":
             assert iters == ref["iters"]
             assert abs(norm - ref["norm"]) <= 1e-10 * np.linalg.norm(b)
-            assert abs(maxerr - np.abs(ref["x"] - xt).max()) <= 1e-8
+            want = np.abs(ref["x"] - xt).max()
+            assert abs(maxerr - want) <= 1e-4 * want + 1e-14  # printed with 5 digits
         else:
             assert abs(iters - ref["iters"]) <= 1
             assert abs(norm - ref["norm"]) <= 1e-12 * ref["hist"][0]
