@@ -19,7 +19,7 @@ OK, NOT_CONVERGED = 0, 1
 ERR = {-1: "ARG", -2: "HIP", -3: "NO_DEVICE", -4: "BAD_MATRIX", -5: "BAD_DIAG", -6: "BREAKDOWN",
        -7: "NONFINITE", -8: "OOM", -9: "REF_UNDEFINED", -10: "STATE", -11: "FILE"}
 ERR_BAD_MATRIX, ERR_REF_UNDEFINED, ERR_FILE = -4, -9, -11
-FMT_CSR, FMT_JAD, FMT_COO = 1, 2, 3
+FMT_CSR, FMT_JAD, FMT_COO, FMT_DIA = 1, 2, 3, 4
 MEM_HOST, MEM_DEVICE = 0, 1
 X0_ONE, X0_ZERO, X0_USER = 0, 1, 2
 TOL_ABS, TOL_REL_B, TOL_REL_R0 = 0, 1, 2
@@ -52,7 +52,7 @@ EXPORTS = [
     "cel_gpu_elm_create", "cel_gpu_elm_row_start", "cel_gpu_elm_insert", "cel_gpu_elm_reinsert",
     "cel_gpu_elm_row_completed", "cel_gpu_elm_matrix_completed", "cel_gpu_elm_set_solver",
     "cel_gpu_elm_solve", "cel_gpu_elm_resolve", "cel_gpu_elm_collect_solution", "cel_gpu_elm_stats",
-    "cel_gpu_elm_matrix", "cel_gpu_elm_destroy",
+    "cel_gpu_elm_matrix", "cel_gpu_elm_destroy", "cel_gpu_mat_to_dia", "cel_gpu_mat_dia_info",
 ]
 
 
@@ -292,6 +292,17 @@ class Matrix:
         rc = lib().cel_gpu_mat_to_jad(self.h, mode)
         _check(rc, allow=(OK, -9) if allow_undefined else (OK,))
         return rc
+
+    def to_dia(self):
+        """Diagonal storage for the CSR/PCG kernels (bit-identical products)."""
+        _check(lib().cel_gpu_mat_to_dia(self.h))
+        return self.dia_info()
+
+    def dia_info(self):
+        nd = C.c_int()
+        offs = (C.c_int * 32)()
+        _check(lib().cel_gpu_mat_dia_info(self.h, C.byref(nd), offs))
+        return list(offs[: nd.value])
 
     def get_jad(self):
         mm, nd, nw = C.c_int(), C.c_int(), C.c_longlong()

@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcel_gpu.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["capi.hip", "spmv.hip", "jad.hip", "pcg.hip", "batch.hip", "coo.hip", "gmres.hip", "mmio.cpp", "elm.cpp"]
+SOURCES = ["capi.hip", "spmv.hip", "jad.hip", "pcg.hip", "batch.hip", "coo.hip", "gmres.hip", "dia.hip", "mmio.cpp", "elm.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -398,6 +398,10 @@ extern "C" int cel_gpu_mat_scale_diagonal(cel_gpu_mat* A)
     M.vals32.release();
     M.jad_mode = -1;
     M.jad_vals32.release();
+    M.dia_nd = 0;
+    M.dia_vals.release();
+    M.dia_vals32.release();
+    M.dia_mask.release();
     CEL_HIP(hipStreamSynchronize(rt().stream));
     return CEL_OK;
 }
@@ -446,9 +450,12 @@ static int spmv_impl(Mat& M, int format, const T* x, T* y, int accumulate, int m
                      const double* ghost, bool distr)
 {
     CEL_TRY(ensure_init());
-    if (format != CEL_FMT_CSR && format != CEL_FMT_JAD) return fail(CEL_ERR_ARG, "format must be CSR or JAD");
+    if (format != CEL_FMT_CSR && format != CEL_FMT_JAD && format != CEL_FMT_DIA)
+        return fail(CEL_ERR_ARG, "format must be CSR, JAD or DIA");
     if (format == CEL_FMT_JAD && (M.jad_mode < 0 || M.jad_ndef != M.jad_mmax))
         return fail(CEL_ERR_STATE, "no complete JAD arrays (convert first)");
+    if (format == CEL_FMT_DIA && M.dia_nd == 0)
+        return fail(CEL_ERR_STATE, "no DIA form (cel_gpu_mat_to_dia first)");
     constexpr bool f32 = sizeof(T) == 4;
     if (f32) {
         if (format == CEL_FMT_CSR && !M.vals32.p) {
@@ -479,9 +486,11 @@ static int spmv_impl(Mat& M, int format, const T* x, T* y, int accumulate, int m
     }
     if constexpr (f32) {
         if (format == CEL_FMT_CSR) CEL_TRY(launch_csr_spmv_f32(M, (const float*)dx, (float*)dy, accumulate));
+        else if (format == CEL_FMT_DIA) CEL_TRY(launch_dia_spmv_f32(M, (const float*)dx, (float*)dy, accumulate));
         else CEL_TRY(launch_jad_spmv_f32(M, (const float*)dx, (float*)dy, accumulate));
     } else {
         if (format == CEL_FMT_CSR) CEL_TRY(launch_csr_spmv(M, (const double*)dx, (double*)dy, accumulate));
+        else if (format == CEL_FMT_DIA) CEL_TRY(launch_dia_spmv(M, (const double*)dx, (double*)dy, accumulate));
         else CEL_TRY(launch_jad_spmv(M, (const double*)dx, (double*)dy, accumulate));
         if (distr && M.halo_nrows) CEL_TRY(launch_halo_spmv(M, dg, (double*)dy));
     }
@@ -523,13 +532,17 @@ extern "C" int cel_gpu_spmv_time(cel_gpu_mat* A, int format, int precision, cons
     auto once = [&]() -> int {
         if (precision == CEL_PREC_FP64) {
             if (format == CEL_FMT_CSR) return launch_csr_spmv(M, (const double*)x, (double*)y, 0);
+            if (format == CEL_FMT_DIA) return launch_dia_spmv(M, (const double*)x, (double*)y, 0);
             return launch_jad_spmv(M, (const double*)x, (double*)y, 0);
         }
         if (format == CEL_FMT_CSR) return launch_csr_spmv_f32(M, (const float*)x, (float*)y, 0);
+        if (format == CEL_FMT_DIA) return launch_dia_spmv_f32(M, (const float*)x, (float*)y, 0);
         return launch_jad_spmv_f32(M, (const float*)x, (float*)y, 0);
     };
     if (format == CEL_FMT_JAD && (M.jad_mode < 0 || M.jad_ndef != M.jad_mmax))
         return fail(CEL_ERR_STATE, "no complete JAD arrays");
+    if (format == CEL_FMT_DIA && M.dia_nd == 0) return fail(CEL_ERR_STATE, "no DIA form");
+    if (format == CEL_FMT_DIA && precision != CEL_PREC_FP64) CEL_TRY(dia_prepare_f32(M));
     if (precision != CEL_PREC_FP64) {
         if (format == CEL_FMT_CSR && !M.vals32.p) {
             CEL_TRY(M.vals32.alloc(M.nnz > 0 ? M.nnz : 1));

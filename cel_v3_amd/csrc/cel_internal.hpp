@@ -117,6 +117,13 @@ struct Mat {
     DevBuf<int> jad_perm, jad_begin, jad_cols;
     DevBuf<double> jad_vals;
     DevBuf<float> jad_vals32;
+
+    // DIA (cel_gpu_mat_to_dia): values [ndiag][n], presence mask per row
+    int dia_nd = 0;                 // 0: no DIA form
+    int dia_off[32] = {};           // ascending column - row offsets
+    DevBuf<double> dia_vals;
+    DevBuf<float> dia_vals32;
+    DevBuf<uint32_t> dia_mask;
     std::vector<int> h_jad_begin;  // host copy (0-based starts, mmax+1)
 
     // halo (distributed block split of one rank)
@@ -159,6 +166,9 @@ int launch_csr_spmv(const Mat& A, const double* x, double* y, int accumulate);
 int launch_csr_spmv_f32(const Mat& A, const float* x, float* y, int accumulate);
 int launch_jad_spmv(const Mat& A, const double* x, double* y, int accumulate);
 int launch_jad_spmv_f32(const Mat& A, const float* x, float* y, int accumulate);
+int launch_dia_spmv(const Mat& A, const double* x, double* y, int accumulate);
+int launch_dia_spmv_f32(Mat& A, const float* x, float* y, int accumulate);
+int dia_prepare_f32(Mat& A);
 int launch_halo_spmv(const Mat& A, const double* ghost, double* y);
 int launch_face_pack(const Mat& A, const double* x, double* sendbuf);
 int launch_extract_diag(Mat& A);

@@ -1,0 +1,201 @@
+// dia.hip -- DIA form of a CSR block and its SpMV (no reference counterpart).
+//
+// A block whose entries lie on a few diagonals (structured-grid stencils)
+// is stored per diagonal: vals[d * n + i] is the entry of row i in column
+// i + off[d] (0 where absent), bit d of mask[i] tells whether it exists.
+// Offsets ascend, so a row sum over d visits the row's entries in the CSR
+// column order; absent entries are skipped (not added as zeros: 0 * x is
+// not +0 for every x), so the product is bit-identical to the CSR one,
+// without column indices and with x read contiguously per diagonal.
+#include <hipcub/hipcub.hpp>
+
+#include "cel_internal.hpp"
+#include "spmv_device.hpp"
+
+namespace cel {
+
+constexpr int kMaxDiag = 32;
+
+namespace {
+
+int grid_of(long long n)
+{
+    long long g = (n + kBlock - 1) / kBlock;
+    long long cap = (long long)rt().num_cus * 16;
+    return (int)(g < 1 ? 1 : g > cap ? cap : g);
+}
+
+// off[j] = col - row for every entry; flags rows whose columns do not
+// strictly ascend.
+__global__ void __launch_bounds__(kBlock)
+k_dia_offsets(int n, const int* __restrict__ xadj, const int* __restrict__ cols,
+              int* __restrict__ off, int* __restrict__ bad)
+{
+    int any = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        int prev = -1;
+        for (int j = xadj[i]; j < xadj[i + 1]; ++j) {
+            const int c = cols[j];
+            any |= c <= prev;
+            prev = c;
+            off[j] = c - (int)i;
+        }
+    }
+    if (any) atomicOr(bad, 1);
+}
+
+struct Offsets {
+    int nd;
+    int off[kMaxDiag];
+};
+
+__global__ void __launch_bounds__(kBlock)
+k_dia_fill(int n, const int* __restrict__ xadj, const int* __restrict__ cols,
+           const double* __restrict__ vals, Offsets O, double* __restrict__ dv,
+           uint32_t* __restrict__ mask)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        unsigned m = 0;
+        for (int j = xadj[i]; j < xadj[i + 1]; ++j) {
+            const int o = cols[j] - (int)i;
+            int d = 0;
+            while (d < O.nd - 1 && O.off[d] != o) ++d;
+            dv[(size_t)d * n + i] = vals[j];
+            m |= 1u << d;
+        }
+        mask[i] = m;
+    }
+}
+
+template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+k_dia_spmv(int n, Offsets O, const VT* __restrict__ dv, const uint32_t* __restrict__ mask,
+           const VT* __restrict__ x, VT* __restrict__ y, int accumulate)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const unsigned m = mask[i];
+        VT s = VT(0);
+        for (int d = 0; d < O.nd; ++d) {
+            long long c = i + O.off[d];
+            c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+            const VT prod = ld_stream(dv + (size_t)d * n + i) * x[c];
+            s = ((m >> d) & 1u) ? s + prod : s;
+        }
+        y[i] = accumulate ? y[i] + s : s;
+    }
+}
+
+Offsets offsets_of(const Mat& A)
+{
+    Offsets O{};
+    O.nd = A.dia_nd;
+    for (int d = 0; d < A.dia_nd; ++d) O.off[d] = A.dia_off[d];
+    return O;
+}
+
+}  // namespace
+
+int launch_dia_spmv(const Mat& A, const double* x, double* y, int accumulate)
+{
+    if (A.n == 0) return CEL_OK;
+    hipLaunchKernelGGL(k_dia_spmv<double>, dim3(grid_of(A.n)), dim3(kBlock), 0, rt().stream, A.n,
+                       offsets_of(A), A.dia_vals.p, A.dia_mask.p, x, y, accumulate);
+    CEL_HIP(hipGetLastError());
+    return CEL_OK;
+}
+
+int dia_prepare_f32(Mat& A)
+{
+    if (A.dia_vals32.p || A.dia_nd == 0) return CEL_OK;
+    const size_t cnt = (size_t)A.dia_nd * (size_t)A.n;
+    CEL_TRY(A.dia_vals32.alloc(cnt > 0 ? cnt : 1));
+    return launch_to_f32(A.dia_vals.p, A.dia_vals32.p, cnt);
+}
+
+int launch_dia_spmv_f32(Mat& A, const float* x, float* y, int accumulate)
+{
+    if (A.n == 0) return CEL_OK;
+    CEL_TRY(dia_prepare_f32(A));
+    hipLaunchKernelGGL(k_dia_spmv<float>, dim3(grid_of(A.n)), dim3(kBlock), 0, rt().stream, A.n,
+                       offsets_of(A), A.dia_vals32.p, A.dia_mask.p, x, y, accumulate);
+    CEL_HIP(hipGetLastError());
+    return CEL_OK;
+}
+
+}  // namespace cel
+
+using namespace cel;
+
+extern "C" int cel_gpu_mat_to_dia(cel_gpu_mat* A)
+{
+    CEL_TRY(ensure_init());
+    if (!A) return fail(CEL_ERR_ARG, "null matrix");
+    Mat& M = A->m;
+    if (M.dia_nd > 0) return CEL_OK;
+    if (M.n == 0 || M.nnz == 0) return fail(CEL_ERR_BAD_MATRIX, "an empty block has no DIA form");
+    hipStream_t st = rt().stream;
+    const long long nnz = M.nnz;
+    DevBuf<int> off, sorted, uniq, counts, nruns, flag;
+    CEL_TRY(off.alloc(nnz));
+    CEL_TRY(flag.alloc(1));
+    CEL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_dia_offsets, dim3(grid_of(M.n)), dim3(kBlock), 0, st, M.n, M.xadj.p,
+                       M.cols.p, off.p, flag.p);
+    CEL_HIP(hipGetLastError());
+    int bad = 0;
+    CEL_HIP(hipMemcpyAsync(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    CEL_HIP(hipStreamSynchronize(st));
+    if (bad) return fail(CEL_ERR_BAD_MATRIX, "columns not strictly ascending in every row: no DIA form");
+    // distinct offsets: sort, then run-length encode
+    CEL_TRY(sorted.alloc(nnz));
+    CEL_TRY(uniq.alloc(nnz));
+    CEL_TRY(counts.alloc(nnz));
+    CEL_TRY(nruns.alloc(1));
+    size_t b1 = 0, b2 = 0;
+    CEL_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, b1, off.p, sorted.p, (int)nnz, 0, 32, st));
+    CEL_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, b2, sorted.p, uniq.p, counts.p, nruns.p,
+                                                  (int)nnz, st));
+    DevBuf<unsigned char> temp;
+    CEL_TRY(temp.alloc(std::max(b1, b2) > 0 ? std::max(b1, b2) : 1));
+    // signed keys: radix sort orders int by value (two's complement handled by hipCUB)
+    CEL_HIP(hipcub::DeviceRadixSort::SortKeys(temp.p, b1, off.p, sorted.p, (int)nnz, 0, 32, st));
+    CEL_HIP(hipcub::DeviceRunLengthEncode::Encode(temp.p, b2, sorted.p, uniq.p, counts.p, nruns.p,
+                                                  (int)nnz, st));
+    int nd = 0;
+    CEL_HIP(hipMemcpyAsync(&nd, nruns.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    CEL_HIP(hipStreamSynchronize(st));
+    if (nd > kMaxDiag)
+        return fail(CEL_ERR_BAD_MATRIX, std::to_string(nd) + " diagonals: more than the DIA form holds (32)");
+    if ((long long)nd * M.n > 2 * nnz)
+        return fail(CEL_ERR_BAD_MATRIX, "DIA would store more than twice nnz slots");
+    int hoff[kMaxDiag] = {};
+    CEL_HIP(hipMemcpyAsync(hoff, uniq.p, sizeof(int) * nd, hipMemcpyDeviceToHost, st));
+    CEL_HIP(hipStreamSynchronize(st));
+    const size_t cnt = (size_t)nd * (size_t)M.n;
+    CEL_TRY(M.dia_vals.alloc(cnt));
+    CEL_TRY(M.dia_mask.alloc(M.n));
+    CEL_HIP(hipMemsetAsync(M.dia_vals.p, 0, sizeof(double) * cnt, st));
+    Offsets O{};
+    O.nd = nd;
+    for (int d = 0; d < nd; ++d) O.off[d] = hoff[d];
+    hipLaunchKernelGGL(k_dia_fill, dim3(grid_of(M.n)), dim3(kBlock), 0, st, M.n, M.xadj.p, M.cols.p,
+                       M.vals.p, O, M.dia_vals.p, M.dia_mask.p);
+    CEL_HIP(hipGetLastError());
+    CEL_HIP(hipStreamSynchronize(st));
+    M.dia_nd = nd;
+    for (int d = 0; d < nd; ++d) M.dia_off[d] = hoff[d];
+    M.dia_vals32.release();
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_mat_dia_info(const cel_gpu_mat* A, int* ndiag, int* offsets)
+{
+    if (!A) return fail(CEL_ERR_ARG, "null matrix");
+    if (ndiag) *ndiag = A->m.dia_nd;
+    if (offsets)
+        for (int d = 0; d < A->m.dia_nd; ++d) offsets[d] = A->m.dia_off[d];
+    return CEL_OK;
+}

@@ -121,6 +121,12 @@ struct MatView {
     const int *jbegin, *jcols, *jperm;
     const double* jvals;
     const float* jvals32;
+    // DIA: values [nd][n], presence mask, ascending offsets
+    const double* dvals;
+    const float* dvals32;
+    const uint32_t* dmask;
+    int nd;
+    int doff[32];
     // halo (FP64 only)
     const int *hrow_tile, *hrows, *hptr, *hcols;
     const double* hvals;
@@ -133,6 +139,32 @@ template <>
 __device__ __forceinline__ const double* csr_vals<double>(const MatView& A) { return A.vals; }
 template <>
 __device__ __forceinline__ const float* csr_vals<float>(const MatView& A) { return A.vals32; }
+template <typename VT>
+__device__ __forceinline__ const VT* dia_vals(const MatView& A);
+template <>
+__device__ __forceinline__ const double* dia_vals<double>(const MatView& A) { return A.dvals; }
+template <>
+__device__ __forceinline__ const float* dia_vals<float>(const MatView& A) { return A.dvals32; }
+
+// DIA row sum: the row's entries in ascending offset (= CSR column) order,
+// absent ones skipped; x[row] returned through xr.
+template <typename VT>
+__device__ __forceinline__ VT dia_row(const MatView& A, int row, const VT* __restrict__ x, VT& xr)
+{
+    const VT* dv = dia_vals<VT>(A);
+    const unsigned m = A.dmask[row];
+    const long long n = A.n;
+    VT s = VT(0);
+    for (int d = 0; d < A.nd; ++d) {
+        long long c = (long long)row + A.doff[d];
+        c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+        const VT prod = ld_stream(dv + (size_t)d * (size_t)n + row) * x[c];
+        s = ((m >> d) & 1u) ? s + prod : s;
+    }
+    xr = x[row];
+    return s;
+}
+
 template <typename VT>
 __device__ __forceinline__ const VT* jad_vals(const MatView& A);
 template <>
@@ -362,6 +394,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
                 __syncthreads();
             }
             if (row < A.n) {
+                v[row] = s;
+                double prod = double(s) * double(pr);
+                acc = acc + prod;
+            }
+        }
+    } else if constexpr (FMT == CEL_FMT_DIA) {
+        for (long long t = t0; t < t1; ++t) {
+            const int row = (int)t * kTileRows + threadIdx.x;
+            if (row < A.n) {
+                VT pr;
+                VT s = dia_row<VT>(A, row, p, pr);
+                s = VT(0) + s;
                 v[row] = s;
                 double prod = double(s) * double(pr);
                 acc = acc + prod;
@@ -678,9 +722,12 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     const bool mixed = prm.precision == CEL_PREC_MIXED;
     const int fmt = prm.format;
     const bool halo = prm.use_halo != 0;
-    if (fmt != CEL_FMT_CSR && fmt != CEL_FMT_JAD) return fail(CEL_ERR_ARG, "format must be CSR or JAD");
+    if (fmt != CEL_FMT_CSR && fmt != CEL_FMT_JAD && fmt != CEL_FMT_DIA)
+        return fail(CEL_ERR_ARG, "format must be CSR, JAD or DIA");
     if (fmt == CEL_FMT_JAD && (A.jad_mode < 0 || A.jad_ndef != A.jad_mmax))
         return fail(CEL_ERR_STATE, "JAD requested but the matrix has no complete JAD arrays");
+    if (fmt == CEL_FMT_DIA && A.dia_nd == 0)
+        return fail(CEL_ERR_STATE, "DIA requested but the matrix has no DIA form");
     if (halo && (mixed || fmt != CEL_FMT_CSR))
         return fail(CEL_ERR_ARG, "halo blocks are applied by the FP64 CSR form only");
     if (!A.has_jac) return fail(CEL_ERR_STATE, "cel_gpu_mat_set_jacobi has not been called");
@@ -697,8 +744,17 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     // grids: K1 variants that must agree (with and without halo) share one
     // grid; the residual kernels share one; K2 and K3 share one.
     int G1, Gr, Gv;
+    // the DIA K1 runs on the CSR K1's grid (the same rows per block, so the
+    // same partial sums); the residual kernels of a DIA solve use the CSR form
+    const int rfmt = fmt == CEL_FMT_DIA ? CEL_FMT_CSR : fmt;
     if (!mixed) {
-        if (fmt == CEL_FMT_CSR) {
+        if (fmt == CEL_FMT_DIA) {
+            G1 = common_grid({(const void*)k_pcg_spmv<double, CEL_FMT_CSR, false>,
+                              (const void*)k_pcg_spmv<double, CEL_FMT_CSR, true>,
+                              (const void*)k_pcg_spmv<double, CEL_FMT_DIA, false>}, ntiles);
+            Gr = common_grid({(const void*)k_residual<double, CEL_FMT_CSR, false, false>,
+                              (const void*)k_residual<double, CEL_FMT_CSR, true, false>}, ntiles);
+        } else if (fmt == CEL_FMT_CSR) {
             G1 = common_grid({(const void*)k_pcg_spmv<double, CEL_FMT_CSR, false>,
                               (const void*)k_pcg_spmv<double, CEL_FMT_CSR, true>}, ntiles);
             Gr = common_grid({(const void*)k_residual<double, CEL_FMT_CSR, false, false>,
@@ -708,7 +764,12 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
             Gr = common_grid({(const void*)k_residual<double, CEL_FMT_JAD, false, false>}, ntiles);
         }
     } else {
-        if (fmt == CEL_FMT_CSR) {
+        if (fmt == CEL_FMT_DIA) {
+            G1 = common_grid({(const void*)k_pcg_spmv<float, CEL_FMT_CSR, false>,
+                              (const void*)k_pcg_spmv<float, CEL_FMT_DIA, false>}, ntiles);
+            Gr = common_grid({(const void*)k_residual<float, CEL_FMT_CSR, false, false>,
+                              (const void*)k_residual<float, CEL_FMT_CSR, false, true>}, ntiles);
+        } else if (fmt == CEL_FMT_CSR) {
             G1 = common_grid({(const void*)k_pcg_spmv<float, CEL_FMT_CSR, false>}, ntiles);
             Gr = common_grid({(const void*)k_residual<float, CEL_FMT_CSR, false, false>,
                               (const void*)k_residual<float, CEL_FMT_CSR, false, true>}, ntiles);
@@ -727,7 +788,8 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
 
     // matrix data for the mixed inner operator (once per matrix)
     if (mixed) {
-        if (fmt == CEL_FMT_CSR && !A.vals32.p) {
+        if (fmt == CEL_FMT_DIA) CEL_TRY(dia_prepare_f32(A));
+        if ((fmt == CEL_FMT_CSR || fmt == CEL_FMT_DIA) && !A.vals32.p) {
             CEL_TRY(A.vals32.alloc(A.nnz > 0 ? A.nnz : 1));
             CEL_TRY(launch_to_f32(A.vals.p, A.vals32.p, A.nnz));
         }
@@ -806,6 +868,11 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     V.jperm = A.jad_perm.p;
     V.jvals = A.jad_vals.p;
     V.jvals32 = A.jad_vals32.p;
+    V.dvals = A.dia_vals.p;
+    V.dvals32 = A.dia_vals32.p;
+    V.dmask = A.dia_mask.p;
+    V.nd = A.dia_nd;
+    for (int d = 0; d < A.dia_nd; ++d) V.doff[d] = A.dia_off[d];
     if (halo) {
         V.hrow_tile = W.hrow_tile.p;
         V.hrows = A.halo_rows.p;
@@ -832,17 +899,17 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
 #define RES_ARGS(OT, R, P_)                                                                \
     V, dx, db, A.jac.p, A.jac32.p, (OT*)(R), (OT*)(P_), W.e32.p, W.rpart.p, C0, C0, P
     if (!mixed) {
-        if (fmt == CEL_FMT_CSR && halo)
+        if (rfmt == CEL_FMT_CSR && halo)
             hipLaunchKernelGGL((k_residual<double, CEL_FMT_CSR, true, false>), dim3(Gr), dim3(kBlock), 0, s,
                                RES_ARGS(double, W.r.p, W.p.p));
-        else if (fmt == CEL_FMT_CSR)
+        else if (rfmt == CEL_FMT_CSR)
             hipLaunchKernelGGL((k_residual<double, CEL_FMT_CSR, false, false>), dim3(Gr), dim3(kBlock), 0,
                                s, RES_ARGS(double, W.r.p, W.p.p));
         else
             hipLaunchKernelGGL((k_residual<double, CEL_FMT_JAD, false, false>), dim3(Gr), dim3(kBlock), 0,
                                s, RES_ARGS(double, W.r.p, W.p.p));
     } else {
-        if (fmt == CEL_FMT_CSR)
+        if (rfmt == CEL_FMT_CSR)
             hipLaunchKernelGGL((k_residual<float, CEL_FMT_CSR, false, false>), dim3(Gr), dim3(kBlock), 0, s,
                                RES_ARGS(float, W.r32.p, W.p32.p));
         else
@@ -882,6 +949,9 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
             else if (fmt == CEL_FMT_CSR)
                 hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_CSR, false>), dim3(G1), dim3(kBlock), 0, s, V,
                                    W.p.p, W.v.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
+            else if (fmt == CEL_FMT_DIA)
+                hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_DIA, false>), dim3(G1), dim3(kBlock), 0, s, V,
+                                   W.p.p, W.v.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
             else
                 hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_JAD, false>), dim3(G1), dim3(kBlock), 0, s, V,
                                    W.p.p, W.v.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
@@ -894,6 +964,9 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
             if (fmt == CEL_FMT_CSR)
                 hipLaunchKernelGGL((k_pcg_spmv<float, CEL_FMT_CSR, false>), dim3(G1), dim3(kBlock), 0, s, V,
                                    W.p32.p, W.v32.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
+            else if (fmt == CEL_FMT_DIA)
+                hipLaunchKernelGGL((k_pcg_spmv<float, CEL_FMT_DIA, false>), dim3(G1), dim3(kBlock), 0, s, V,
+                                   W.p32.p, W.v32.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
             else
                 hipLaunchKernelGGL((k_pcg_spmv<float, CEL_FMT_JAD, false>), dim3(G1), dim3(kBlock), 0, s, V,
                                    W.p32.p, W.v32.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
@@ -902,7 +975,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
                                A.jac32.p, W.part2.p, W.part1.p, G1, C1, C2);
             hipLaunchKernelGGL((k_pcg_update_xp<float, float, true>), dim3(Gv), dim3(kBlock), 0, s, n_ll,
                                W.e32.p, W.p32.p, W.r32.p, A.jac32.p, dx, W.part2.p, Gv, C2, C3, P);
-            if (fmt == CEL_FMT_CSR)
+            if (rfmt == CEL_FMT_CSR)
                 hipLaunchKernelGGL((k_residual<float, CEL_FMT_CSR, false, true>), dim3(Gr), dim3(kBlock), 0, s,
                                    V, dx, db, A.jac.p, A.jac32.p, W.r32.p, W.p32.p, W.e32.p, W.rpart.p, C3,
                                    C0, P);

@@ -50,6 +50,8 @@ extern "C" {
 #define CEL_FMT_CSR 1
 #define CEL_FMT_JAD 2
 #define CEL_FMT_COO 3
+#define CEL_FMT_DIA 4  /* diagonal storage of a CSR block whose entries lie on at most
+                          32 diagonals (no reference counterpart; see cel_gpu_mat_to_dia) */
 
 #define CEL_MEM_HOST 0
 #define CEL_MEM_DEVICE 1
@@ -238,6 +240,20 @@ int cel_gpu_mat_to_jad(cel_gpu_mat* A, int mode);
  * (entries written).  *ndefined = diagonals whose data is defined. */
 int cel_gpu_mat_get_jad(const cel_gpu_mat* A, int* perm, int* begin, int* cols, double* vals,
                         int* mmax, int* ndefined, long long* nnz_written);
+
+/* ---- CSR -> DIA on the device (no reference counterpart) --------------- */
+/* For a block whose entries lie on at most 32 diagonals (offsets col - row,
+ * e.g. 7/19/27-point stencils) and whose rows hold strictly ascending
+ * columns: values stored per diagonal (diagonal-major, n entries each,
+ * absent entries marked in a per-row mask), offsets ascending.  A DIA row
+ * sum visits the row's entries in the CSR column order and skips absent
+ * ones, so y = A x is bit-identical to the CSR product, with no column
+ * indices read and x read contiguously per diagonal.  Fails with
+ * CEL_ERR_BAD_MATRIX (and builds nothing) for more than 32 diagonals,
+ * unsorted or repeated columns, or more than twice nnz stored slots. */
+int cel_gpu_mat_to_dia(cel_gpu_mat* A);
+/* Number of diagonals and their offsets (ascending; up to 32). */
+int cel_gpu_mat_dia_info(const cel_gpu_mat* A, int* ndiag, int* offsets);
 
 /* ---- diagonal, Jacobi vector, row scaling ------------------------------ */
 /* cel_sp_mat_distr_set_jacobi (src/fspmtdistr/cel_sp_mat_distr_format_module.f90:149-203):

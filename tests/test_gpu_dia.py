@@ -1,0 +1,124 @@
+"""DIA form (cel_gpu_mat_to_dia; no reference counterpart) of CSR blocks
+whose entries lie on a few diagonals.  Its products must be bit-identical to
+the CSR products -- hence to the oracle's restatement of the reference CSR
+kernel (src/fop/cel_fop_mv_module.f90:215-229) -- and a PCG solve with it
+bit-identical to the CSR solve, FP64 and mixed."""
+import numpy as np
+import pytest
+
+import oracle as O
+from test_oracle import convdiff_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def cases():
+    rng = np.random.default_rng(61)
+    kappa = np.exp(rng.normal(0.0, 0.7, 14 * 11 * 9))
+    gold = np.load(O.__file__.replace("oracle/__init__.py", "tests/golden/ref_generator.npz"))
+    out = [("poisson20x17x13", O.gen_poisson7(20, 17, 13)),
+           ("diffusion14x11x9", O.gen_diffusion7(14, 11, 9, kappa)),
+           ("convdiff12x10x8", convdiff_csr(12, 10, 8)),
+           ("poisson1x1x5", O.gen_poisson7(1, 1, 5))]
+    for base in ("s27_5x5x5_r0", "s19_4x5x6_r0"):  # the reference generator's matrices
+        n = int(gold[base + "_xt"].size)
+        r, c, v = O.coo_sort_rowcol(gold[base + "_rows"] + 1, gold[base + "_cols"] + 1,
+                                    gold[base + "_vals"])  # stored 0-based
+        out.append((f"refgen_{base}", O.CSR(O.coo_to_csr(n, r), c, v)))
+    return out
+
+
+def dev(gpu, A):
+    return gpu.Matrix.from_csr(A.xadj, A.cols, A.vals)
+
+
+@pytest.mark.parametrize("name,A", cases(), ids=[c[0] for c in cases()])
+def test_dia_spmv_is_the_csr_product(gpu, name, A):
+    print(f"case {name} n={A.n} nnz={A.nnz}", flush=True)
+    M = dev(gpu, A)
+    offs = M.to_dia()
+    assert offs == sorted(offs) and len(offs) <= 32
+    rng = np.random.default_rng(62)
+    x = rng.normal(size=A.n)
+    np.testing.assert_array_equal(M.spmv(x, fmt=gpu.FMT_DIA), O.spmv_csr(A, x))
+    y0 = rng.normal(size=A.n)
+    np.testing.assert_array_equal(M.spmv(x, y0, fmt=gpu.FMT_DIA, accumulate=True), O.spmv_csr(A, x, y0))
+    x32 = x.astype(np.float32)
+    np.testing.assert_array_equal(M.spmv_f32(x32, fmt=gpu.FMT_DIA), O.spmv_csr_f32(A, x32))
+
+
+def test_dia_skips_absent_entries_even_for_non_finite_x(gpu):
+    """Boundary rows read a clamped x for their absent diagonals and must
+    discard it: Inf and NaN in x reach exactly the rows CSR sends them to."""
+    A = O.gen_poisson7(9, 8, 7)
+    M = dev(gpu, A)
+    M.to_dia()
+    x = np.random.default_rng(63).normal(size=A.n)
+    x[0], x[-1], x[9] = np.inf, np.nan, -np.inf
+    np.testing.assert_array_equal(M.spmv(x, fmt=gpu.FMT_DIA), M.spmv(x))
+
+
+def test_dia_refuses_what_it_cannot_hold(gpu):
+    rng = np.random.default_rng(64)
+    n = 200
+    rows = np.repeat(np.arange(n), 5)
+    cols = rng.integers(0, n, rows.size)
+    keys = np.unique(rows.astype(np.int64) * n + cols)
+    r, c = keys // n, keys % n
+    xadj = np.concatenate([[0], np.cumsum(np.bincount(r, minlength=n))]) + 1
+    M = gpu.Matrix.from_csr(xadj, c + 1, rng.normal(size=c.size))
+    with pytest.raises(gpu.CelError) as e:
+        M.to_dia()
+    assert e.value.code == gpu.ERR_BAD_MATRIX
+    with pytest.raises(gpu.CelError):
+        M.spmv(np.ones(n), fmt=gpu.FMT_DIA)
+
+
+def test_scaling_drops_the_dia_form(gpu):
+    A = O.gen_poisson7(10, 9, 8)
+    M = dev(gpu, A)
+    M.to_dia()
+    M.scale_diagonal()
+    x = np.random.default_rng(65).normal(size=A.n)
+    with pytest.raises(gpu.CelError):
+        M.spmv(x, fmt=gpu.FMT_DIA)
+    M.to_dia()
+    np.testing.assert_array_equal(M.spmv(x, fmt=gpu.FMT_DIA), M.spmv(x))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pcg_with_dia_is_bitwise_the_csr_solve(gpu, precision):
+    rng = np.random.default_rng(66)
+    kappa = np.exp(rng.normal(0.0, 0.5, 40 * 36 * 32))
+    M = gpu.Matrix.diffusion7(40, 36, 32, kappa)
+    M.set_jacobi()
+    b = rng.random(M.n)
+    prec = gpu.PREC_FP64 if precision == "fp64" else gpu.PREC_MIXED
+    kw = dict(tol=1e-10, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, precision=prec)
+    a = M.pcg(b, **kw)
+    M.to_dia()
+    d = M.pcg(b, fmt=gpu.FMT_DIA, **kw)
+    assert a["status"] == d["status"] == 0
+    assert (a["iters"], a["outer_iters"]) == (d["iters"], d["outer_iters"])
+    np.testing.assert_array_equal(a["x"], d["x"])
+    assert a["final_norm"] == d["final_norm"]
+
+
+def test_pcg_dia_256_timing(gpu):
+    """The 256^3 bench block: CSR and DIA mixed solves agree bit for bit;
+    prints both times and the K1 means."""
+    M = gpu.Matrix.poisson7(256, 256, 256)
+    M.set_jacobi()
+    xt = np.random.default_rng(1).random(M.n)
+    d_xt = gpu.DeviceArray.from_host(xt)
+    d_b = gpu.DeviceArray(M.n)
+    M.spmv(d_xt, d_b)
+    kw = dict(tol=1e-10, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, precision=gpu.PREC_MIXED,
+              iter_max=20000)
+    a = M.pcg(d_b, **kw)
+    M.to_dia()
+    d = M.pcg(d_b, fmt=gpu.FMT_DIA, **kw)
+    d = M.pcg(d_b, fmt=gpu.FMT_DIA, **kw)
+    print(f"mixed CSR {a['solve_ms']:.1f} ms, DIA {d['solve_ms']:.1f} ms, iters {a['iters']}/{d['iters']}",
+          flush=True)
+    np.testing.assert_array_equal(a["x"].to_host(), d["x"].to_host())
