@@ -2,7 +2,7 @@
 """256^3 Jacobi-PCG time-to-1e-10 on one MI355X: mixed precision vs plain FP64,
 and SpMV bandwidth against the HBM roofline.
 
-    python bench.py [--steps K] [--warmup W] [--size N] [--format csr|jad]
+    python bench.py [--steps K] [--warmup W] [--size N] [--format dia|csr|jad]
 
 One step = one full solve of the 7-point Poisson system on an N^3 grid
 (diag 6, off-diagonal -1, Dirichlet outside), b = A x_true with
@@ -10,8 +10,12 @@ x_true ~ U[0,1) (numpy default_rng(1)), x0 = 0, stopped at
 ||b - A x|| <= tol * ||b||.  Inputs are resident in HBM before the timed
 region starts.  Prints one JSON line on stdout; progress goes to stderr.
 
-The timed mixed and FP64 solves use the same local block.  One more FP64
-solve runs the one-subdomain model of the reference's distributed SpMV: the
+The timed mixed and FP64 solves use the same local block, stored in the DIA
+form (its products and the whole solve are bit-identical to the CSR form:
+the same row sums in the same order, without column indices); the same
+solves with the CSR form are timed beside them and must give the same bits.
+One more FP64 solve runs the one-subdomain model of the reference's
+distributed SpMV (CSR, which carries the halo blocks): the
 256^3 block carries six halo faces (65,536 entries each) and packs its six
 send faces every SpMV; with no neighbour present the received values are
 zero, so it solves the same Dirichlet problem.  Per-kernel times come from
@@ -273,7 +277,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--format", default="csr", choices=["csr", "jad"])
+    ap.add_argument("--format", default="dia", choices=["dia", "csr", "jad"])
     ap.add_argument("--skip-cpu", action="store_true", help="do not time the CPU baseline")
     ap.add_argument("--only", default="all",
                     choices=["all", "fp64", "mixed", "spmv", "profile", "batch", "sort"],
@@ -312,7 +316,12 @@ def main():
     if args.format == "jad":
         M.to_jad()
         fmt = cel.FMT_JAD
-    use_halo = int(args.format == "csr")
+    if args.format == "dia":
+        t0 = time.perf_counter()
+        offs = M.to_dia()
+        log(f"DIA form: {len(offs)} diagonals {offs}, built in {(time.perf_counter() - t0) * 1e3:.1f} ms")
+        fmt = cel.FMT_DIA
+    use_halo = int(args.format in ("csr", "dia"))
     if use_halo:
         for proc, rows, cols in face_blocks(N):
             M.add_halo_block(proc, proc * N ** 3 + 1, N ** 3, rows, cols, -np.ones(rows.size))
@@ -324,10 +333,11 @@ def main():
     bnorm = float(np.linalg.norm(d_b.to_host()))
     log(f"n={M.n} nnz={M.nnz} |b|={bnorm:.6e}")
 
-    def solve(precision, profile=0, halo=0):
-        x = cel.DeviceArray(M.n)
+    def solve(precision, profile=0, halo=0, form=None):
+        # the halo blocks are applied by the CSR K1
+        f = cel.FMT_CSR if halo else (fmt if form is None else form)
         r = M.pcg(d_b, x0=None, tol=args.tol, tol_mode=cel.TOL_REL_B, x0_mode=cel.X0_ZERO,
-                  iter_max=20000, precision=precision, fmt=fmt, profile=profile, use_halo=halo,
+                  iter_max=20000, precision=precision, fmt=f, profile=profile, use_halo=halo,
                   allow_not_converged=False)
         return r
 
@@ -349,11 +359,13 @@ def main():
         ms = cel.stream_copy_time(copy_bytes, reps=2)
         prof = {"copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes,
                 "copy_ms": ms}
-        for name, prec, halo in (("fp64", cel.PREC_FP64, 0), ("fp64_halo", cel.PREC_FP64, use_halo),
-                                 ("mixed", cel.PREC_MIXED, 0)):
-            x = cel.DeviceArray(M.n)
+        runs = [("fp64", cel.PREC_FP64, 0, fmt), ("fp64_halo", cel.PREC_FP64, use_halo, cel.FMT_CSR),
+                ("mixed", cel.PREC_MIXED, 0, fmt)]
+        if fmt == cel.FMT_DIA:
+            runs += [("fp64_csr", cel.PREC_FP64, 0, cel.FMT_CSR), ("mixed_csr", cel.PREC_MIXED, 0, cel.FMT_CSR)]
+        for name, prec, halo, f in runs:
             r = M.pcg(d_b, x0=None, tol=args.tol, tol_mode=cel.TOL_REL_B, x0_mode=cel.X0_ZERO,
-                      iter_max=args.iters, precision=prec, fmt=fmt, use_halo=halo, profile=1)
+                      iter_max=args.iters, precision=prec, fmt=f, use_halo=halo, profile=1)
             prof[name] = {"iters": r["iters"], "spmv_bytes": r["spmv_bytes"],
                           "mean_spmv_ms": r["spmv_ms"] / max(r["spmv_launches"], 1)}
             log(f"profile {name}: {r['iters']} iterations, K1 "
@@ -368,8 +380,12 @@ def main():
         x32 = cel.DeviceArray.from_host(xt.astype(np.float32))
         y32 = cel.DeviceArray(M.n, np.float32)
         spmv = {}
-        for name, f, prec, x, y in (("csr_fp64", cel.FMT_CSR, cel.PREC_FP64, x64, y64),
-                                    ("csr_fp32", cel.FMT_CSR, cel.PREC_MIXED, x32, y32)):
+        forms = [("csr_fp64", cel.FMT_CSR, cel.PREC_FP64, x64, y64),
+                 ("csr_fp32", cel.FMT_CSR, cel.PREC_MIXED, x32, y32)]
+        if fmt == cel.FMT_DIA:
+            forms += [("dia_fp64", cel.FMT_DIA, cel.PREC_FP64, x64, y64),
+                      ("dia_fp32", cel.FMT_DIA, cel.PREC_MIXED, x32, y32)]
+        for name, f, prec, x, y in forms:
             ms, nb = M.spmv_time(x, y, fmt=f, precision=prec, reps=20)
             spmv[name] = {"ms": ms, "bytes": nb, "GBs": nb / ms * 1e-6,
                           "frac_hbm": nb / ms * 1e-6 / HBM_PEAK_GBS}
@@ -425,6 +441,15 @@ def main():
             log(f"step {k + 1}: fp64 {r['solve_ms']:.1f} ms, {r['iters']} iterations")
     cel.lib().cel_gpu_synchronize()
     t_fp64 = time.perf_counter() - t_start
+    # the same solves with the CSR form (like-for-like with the reference's
+    # format); their solutions must equal the DIA ones bit for bit
+    csr_runs = {}
+    if fmt == cel.FMT_DIA and args.only == "all":
+        for name, prec in (("mixed", cel.PREC_MIXED), ("fp64", cel.PREC_FP64)):
+            solve(prec, form=cel.FMT_CSR)  # warm-up
+            rs = [solve(prec, form=cel.FMT_CSR) for _ in range(args.steps)]
+            csr_runs[name] = rs
+            log(f"csr {name}: " + ", ".join(f"{r['solve_ms']:.1f}" for r in rs) + " ms")
 
     # ------------------------------------------------------------ per-kernel timing (untimed)
     # one more solve of each form with HIP events around every K1 launch on
@@ -432,8 +457,12 @@ def main():
     prof = {}
     if args.only in ("all", "mixed"):
         prof["mixed"] = solve(cel.PREC_MIXED, profile=1)
+        if fmt == cel.FMT_DIA:
+            prof["mixed_csr"] = solve(cel.PREC_MIXED, profile=1, form=cel.FMT_CSR)
     if args.only in ("all", "fp64"):
         prof["fp64"] = solve(cel.PREC_FP64, profile=1)
+        if fmt == cel.FMT_DIA:
+            prof["fp64_csr"] = solve(cel.PREC_FP64, profile=1, form=cel.FMT_CSR)
         if use_halo:
             prof["fp64_halo"] = solve(cel.PREC_FP64, profile=1, halo=1)
 
@@ -475,10 +504,27 @@ def main():
     if mixed_runs and fp64_runs:
         res["speedup_mixed_vs_fp64"] = (np.median([r["solve_ms"] for r in fp64_runs]) /
                                         np.median([r["solve_ms"] for r in mixed_runs]))
+    if csr_runs:
+        same = {}
+        for name, runs_ in (("mixed", mixed_runs), ("fp64", fp64_runs)):
+            a, b = runs_[-1]["x"].to_host(), csr_runs[name][-1]["x"].to_host()
+            same[name] = bool(np.array_equal(a, b))
+        res["csr"] = {"note": "the same solves with the CSR form",
+                      "mixed_solve_ms": [r["solve_ms"] for r in csr_runs["mixed"]],
+                      "fp64_solve_ms": [r["solve_ms"] for r in csr_runs["fp64"]],
+                      "solutions_bitwise_equal_to_dia": same}
+        res["dia_speedup_vs_csr"] = {
+            k: float(np.median(res["csr"][k + "_solve_ms"]) /
+                     np.median([r["solve_ms"] for r in (mixed_runs if k == "mixed" else fp64_runs)]))
+            for k in ("mixed", "fp64")}
+        log(f"DIA vs CSR: {res['dia_speedup_vs_csr']}, bitwise equal: {same}")
     # the dominant kernel of the timed (mixed) solve is its FP32 K1; the FP64
     # K1 is reported beside it
-    for key, form, name in (("roofline", "mixed", "k_pcg_spmv<float,CSR> (v = A p fused with v.p)"),
-                            ("roofline_fp64", "fp64", "k_pcg_spmv<double,CSR> (v = A p fused with v.p)"),
+    fname = {cel.FMT_DIA: "DIA", cel.FMT_CSR: "CSR", cel.FMT_JAD: "JAD"}[fmt]
+    for key, form, name in (("roofline", "mixed", f"k_pcg_spmv<float,{fname}> (v = A p fused with v.p)"),
+                            ("roofline_fp64", "fp64", f"k_pcg_spmv<double,{fname}> (v = A p fused with v.p)"),
+                            ("roofline_csr", "mixed_csr", "k_pcg_spmv<float,CSR> (v = A p fused with v.p)"),
+                            ("roofline_csr_fp64", "fp64_csr", "k_pcg_spmv<double,CSR> (v = A p fused with v.p)"),
                             ("roofline_fp64_halo", "fp64_halo",
                              "k_pcg_spmv<double,CSR,halo> (+6 halo faces, one-subdomain model)")):
         if form not in prof:

@@ -146,23 +146,49 @@ __device__ __forceinline__ const double* dia_vals<double>(const MatView& A) { re
 template <>
 __device__ __forceinline__ const float* dia_vals<float>(const MatView& A) { return A.dvals32; }
 
-// DIA row sum: the row's entries in ascending offset (= CSR column) order,
-// absent ones skipped; x[row] returned through xr.
+// DIA row sums of two rows (rb may be past the end: its result is not
+// used): the row's entries in ascending offset (= CSR column) order, absent
+// ones skipped.  The loads of a group of 8 diagonals of both rows are
+// issued before any of their products is added.
 template <typename VT>
-__device__ __forceinline__ VT dia_row(const MatView& A, int row, const VT* __restrict__ x, VT& xr)
+__device__ __forceinline__ void dia_rows2(const MatView& A, int ra, int rb,
+                                          const VT* __restrict__ x, VT& sa, VT& sb)
 {
-    const VT* dv = dia_vals<VT>(A);
-    const unsigned m = A.dmask[row];
-    const long long n = A.n;
-    VT s = VT(0);
-    for (int d = 0; d < A.nd; ++d) {
-        long long c = (long long)row + A.doff[d];
-        c = c < 0 ? 0 : (c >= n ? n - 1 : c);
-        const VT prod = ld_stream(dv + (size_t)d * (size_t)n + row) * x[c];
-        s = ((m >> d) & 1u) ? s + prod : s;
+    // buffer resources: one per diagonal (uniform) and one over x; offsets
+    // are 32-bit bytes, and an x offset outside [0, n) (an absent entry of a
+    // boundary row) reads 0 instead of faulting -- it is skipped anyway
+    const int n = A.n;
+    constexpr int B = (int)sizeof(VT);
+    const int rbc = rb < n ? rb : ra;
+    const unsigned ma = A.dmask[ra], mb = A.dmask[rbc];
+    const auto rx = tile_rsrc(x, n * B);
+    sa = VT(0);
+    sb = VT(0);
+    for (int d0 = 0; d0 < A.nd; d0 += 8) {
+        VT va[8], xa[8], vb[8], xb[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int d = d0 + k;
+            if (d < A.nd) {
+                const auto rv = tile_rsrc(dia_vals<VT>(A) + (size_t)d * (size_t)n, n * B);
+                const int o = A.doff[d];
+                va[k] = buf_load_nt<VT>(rv, ra * B, 0);
+                vb[k] = buf_load_nt<VT>(rv, rbc * B, 0);
+                xa[k] = buf_load<VT>(rx, (ra + o) * B);
+                xb[k] = buf_load<VT>(rx, (rbc + o) * B);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int d = d0 + k;
+            if (d < A.nd) {
+                const VT pa = va[k] * xa[k];
+                const VT pb = vb[k] * xb[k];
+                sa = ((ma >> d) & 1u) ? sa + pa : sa;
+                sb = ((mb >> d) & 1u) ? sb + pb : sb;
+            }
+        }
     }
-    xr = x[row];
-    return s;
 }
 
 template <typename VT>
@@ -400,15 +426,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
             }
         }
     } else if constexpr (FMT == CEL_FMT_DIA) {
-        for (long long t = t0; t < t1; ++t) {
-            const int row = (int)t * kTileRows + threadIdx.x;
-            if (row < A.n) {
-                VT pr;
-                VT s = dia_row<VT>(A, row, p, pr);
-                s = VT(0) + s;
-                v[row] = s;
-                double prod = double(s) * double(pr);
+        // two tiles per step; the partial adds the rows in tile order, as
+        // the CSR K1 does on the same grid
+        for (long long t = t0; t < t1; t += 2) {
+            const int ra = (int)t * kTileRows + threadIdx.x;
+            const int rb = t + 1 < t1 ? ra + kTileRows : A.n;
+            if (ra < A.n) {
+                VT sa, sb;
+                dia_rows2<VT>(A, ra, rb, p, sa, sb);
+                const VT pa = p[ra];
+                sa = VT(0) + sa;
+                v[ra] = sa;
+                double prod = double(sa) * double(pa);
                 acc = acc + prod;
+                if (rb < A.n) {
+                    const VT pb = p[rb];
+                    sb = VT(0) + sb;
+                    v[rb] = sb;
+                    prod = double(sb) * double(pb);
+                    acc = acc + prod;
+                }
             }
         }
     } else {

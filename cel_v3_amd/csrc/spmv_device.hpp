@@ -83,6 +83,20 @@ __device__ __forceinline__ int buf_load_nt<int>(__amdgpu_buffer_rsrc_t r, int vo
     return (int)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2);
 }
 
+// Cached buffer load; an offset outside the resource returns 0.
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int voff);
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int voff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+}
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int voff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
+}
+
 // Software-pipelined CSR tile stream.  fetch(t) issues the loads of tile t
 // (row bounds of every thread and, when the tile fits the stage, its
 // entries into registers); stage() parks them in LDS; rows() sums the

@@ -250,7 +250,8 @@ int cel_gpu_mat_get_jad(const cel_gpu_mat* A, int* perm, int* begin, int* cols, 
  * ones, so y = A x is bit-identical to the CSR product, with no column
  * indices read and x read contiguously per diagonal.  Fails with
  * CEL_ERR_BAD_MATRIX (and builds nothing) for more than 32 diagonals,
- * unsorted or repeated columns, or more than twice nnz stored slots. */
+ * unsorted or repeated columns, more than twice nnz stored slots, or more
+ * than 2^28 rows. */
 int cel_gpu_mat_to_dia(cel_gpu_mat* A);
 /* Number of diagonals and their offsets (ascending; up to 32). */
 int cel_gpu_mat_dia_info(const cel_gpu_mat* A, int* ndiag, int* offsets);
