@@ -38,6 +38,7 @@ struct cel_gpu_elm {
     std::vector<double> vals;
     std::vector<long long> diag_at;  // last diagonal entry of each row (-1: none)
     cel_gpu_mat* A = nullptr;
+    int format = CEL_FMT_CSR;       // CEL_FMT_DIA when the block has that form
     std::vector<double> x;
     cel_pcg_stats last{};
 };
@@ -123,6 +124,9 @@ extern "C" int cel_gpu_elm_matrix_completed(cel_gpu_elm* E)
     if (rc < 0) return rc;
     rc = cel_gpu_mat_set_jacobi(E->A, 1);
     if (rc < 0) return rc;
+    // structured-grid blocks also get the DIA form (same products, no
+    // column indices); any other block keeps CSR
+    E->format = cel_gpu_mat_to_dia(E->A) == CEL_OK ? CEL_FMT_DIA : CEL_FMT_CSR;
     E->completed = true;
     // :431-434 the inserted arrays are released
     std::vector<int>().swap(E->rows);
@@ -151,6 +155,7 @@ static int elm_run(cel_gpu_elm* E, const double* b, int scale_b, int* iierr, int
         if (E->solver == CEL_ELM_GMRES) {
             cel_gmres_params p;
             cel_gpu_gmres_params_default(&p);  // eps 1e-8, iter_max 5000, restart 3, x = 1
+            p.format = E->format;
             rc = cel_gpu_gmres_solve(E->A, rhs.data(), E->x.data(), &p, &E->last);
         } else {
             cel_pcg_params p;
@@ -159,6 +164,7 @@ static int elm_run(cel_gpu_elm* E, const double* b, int scale_b, int* iierr, int
             p.tol_mode = CEL_TOL_ABS;
             p.iter_max = 5000;
             p.x0_mode = CEL_X0_ONE;
+            p.format = E->format;
             rc = cel_gpu_pcg_solve(E->A, rhs.data(), E->x.data(), &p, &E->last, nullptr);
         }
     }

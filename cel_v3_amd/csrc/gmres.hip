@@ -184,7 +184,9 @@ struct Gmres {
     int spmv(const double* x, double* y)
     {
         ++launches;
-        return format == CEL_FMT_JAD ? launch_jad_spmv(A, x, y, 0) : launch_csr_spmv(A, x, y, 0);
+        if (format == CEL_FMT_JAD) return launch_jad_spmv(A, x, y, 0);
+        if (format == CEL_FMT_DIA) return launch_dia_spmv(A, x, y, 0);
+        return launch_csr_spmv(A, x, y, 0);
     }
     // r = b - A x, returns ||r||
     int residual(const double* b, const double* x, double* r, double& norm)
@@ -211,6 +213,8 @@ int gmres_solve(Mat& A, const double* b_in, double* x_out, const cel_gmres_param
         return fail(CEL_ERR_STATE, "JAD requested before cel_gpu_mat_to_jad");
     if (prm.format == CEL_FMT_JAD && A.jad_ndef < A.jad_mmax)
         return fail(CEL_ERR_REF_UNDEFINED, "the JAD form of this matrix is undefined in the reference");
+    if (prm.format == CEL_FMT_DIA && A.dia_nd == 0)
+        return fail(CEL_ERR_STATE, "DIA requested before cel_gpu_mat_to_dia");
     if (A.n != A.ncols) return fail(CEL_ERR_ARG, "GMRES needs a square local block");
     const long long n = A.n;
     stats = cel_pcg_stats{};
@@ -382,7 +386,7 @@ extern "C" int cel_gpu_gmres_solve(cel_gpu_mat* A, const double* b, double* x,
     if (params) p = *params;
     else cel_gpu_gmres_params_default(&p);
     if ((p.memspace != CEL_MEM_HOST && p.memspace != CEL_MEM_DEVICE) ||
-        (p.format != CEL_FMT_CSR && p.format != CEL_FMT_JAD) ||
+        (p.format != CEL_FMT_CSR && p.format != CEL_FMT_JAD && p.format != CEL_FMT_DIA) ||
         (p.x0_mode != CEL_X0_ONE && p.x0_mode != CEL_X0_ZERO && p.x0_mode != CEL_X0_USER) ||
         !(p.eps >= 0.0))
         return fail(CEL_ERR_ARG, "bad cel_gmres_params");

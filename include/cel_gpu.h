@@ -393,7 +393,7 @@ typedef struct {
     int restart;    /* m: Arnoldi steps per cycle (1..32); m+1 cycles per outer iteration */
     int x0_mode;    /* CEL_X0_ONE is the reference's x = 1 */
     int memspace;   /* b and x on the host or the device */
-    int format;     /* CEL_FMT_CSR or CEL_FMT_JAD */
+    int format;     /* CEL_FMT_CSR, CEL_FMT_JAD or CEL_FMT_DIA */
 } cel_gmres_params;
 
 /* The element solver's settings (src/felmsolver/cel_elmsolver_module.f90:140-151):
@@ -442,6 +442,8 @@ int cel_gpu_elm_row_completed(cel_gpu_elm* E);
 /* cel_elmSolverMatrixCompleted (:193-439): the reference's COO sort
  * (cel_gpu_coo_sort order, :372), COO->CSR, diagonal captured, rows scaled
  * by 1/sqrt|d_i| and the Jacobi vector sqrt|d|/d (:386-419), all in HBM.
+ * A matrix whose entries lie on at most 32 diagonals is also kept in the
+ * DIA form, which the solves then use (bit-identical products).
  * Insertion after this is CEL_ERR_STATE (the reference frees its arrays). */
 int cel_gpu_elm_matrix_completed(cel_gpu_elm* E);
 /* Solver of the next solves (no reference counterpart; default GMRES). */
