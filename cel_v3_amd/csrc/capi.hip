@@ -402,6 +402,7 @@ extern "C" int cel_gpu_mat_scale_diagonal(cel_gpu_mat* A)
     M.dia_vals.release();
     M.dia_vals32.release();
     M.dia_mask.release();
+    M.dia_mask8.release();
     CEL_HIP(hipStreamSynchronize(rt().stream));
     return CEL_OK;
 }

@@ -123,7 +123,8 @@ struct Mat {
     int dia_off[32] = {};           // ascending column - row offsets
     DevBuf<double> dia_vals;
     DevBuf<float> dia_vals32;
-    DevBuf<uint32_t> dia_mask;
+    DevBuf<uint32_t> dia_mask;      // presence bits per row (more than 8 diagonals)
+    DevBuf<uint8_t> dia_mask8;      // the same in one byte (at most 8 diagonals)
     std::vector<int> h_jad_begin;  // host copy (0-based starts, mmax+1)
 
     // halo (distributed block split of one rank)

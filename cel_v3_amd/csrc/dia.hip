@@ -6,7 +6,8 @@
 // Offsets ascend, so a row sum over d visits the row's entries in the CSR
 // column order; absent entries are skipped (not added as zeros: 0 * x is
 // not +0 for every x), so the product is bit-identical to the CSR one,
-// without column indices and with x read contiguously per diagonal.
+// without column indices and with x read contiguously per diagonal.  The
+// mask is one byte per row when there are at most 8 diagonals.
 #include <hipcub/hipcub.hpp>
 
 #include "cel_internal.hpp"
@@ -53,7 +54,7 @@ struct Offsets {
 __global__ void __launch_bounds__(kBlock)
 k_dia_fill(int n, const int* __restrict__ xadj, const int* __restrict__ cols,
            const double* __restrict__ vals, Offsets O, double* __restrict__ dv,
-           uint32_t* __restrict__ mask)
+           uint32_t* __restrict__ mask, uint8_t* __restrict__ mask8)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
@@ -65,26 +66,28 @@ k_dia_fill(int n, const int* __restrict__ xadj, const int* __restrict__ cols,
             dv[(size_t)d * n + i] = vals[j];
             m |= 1u << d;
         }
-        mask[i] = m;
+        if (mask8) mask8[i] = (uint8_t)m;
+        else mask[i] = m;
     }
 }
 
+// y = A x (accumulate: y += A x); each thread sums rows 2T*256 + tid and
+// 256 rows further for every pair T of 256-row tiles it owns.
 template <typename VT>
 __global__ void __launch_bounds__(kBlock)
 k_dia_spmv(int n, Offsets O, const VT* __restrict__ dv, const uint32_t* __restrict__ mask,
-           const VT* __restrict__ x, VT* __restrict__ y, int accumulate)
+           const uint8_t* __restrict__ mask8, const VT* __restrict__ x, VT* __restrict__ y,
+           int accumulate)
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        const unsigned m = mask[i];
-        VT s = VT(0);
-        for (int d = 0; d < O.nd; ++d) {
-            long long c = i + O.off[d];
-            c = c < 0 ? 0 : (c >= n ? n - 1 : c);
-            const VT prod = ld_stream(dv + (size_t)d * n + i) * x[c];
-            s = ((m >> d) & 1u) ? s + prod : s;
-        }
-        y[i] = accumulate ? y[i] + s : s;
+    const long long pairs = ((long long)n + 2 * kBlock - 1) / (2 * kBlock);
+    for (long long T = blockIdx.x; T < pairs; T += gridDim.x) {
+        const int ra = (int)(T * 2 * kBlock) + threadIdx.x;
+        const int rb = ra + kBlock;
+        if (ra >= n) continue;
+        VT sa, sb;
+        dia_row_pair<VT>(n, O.nd, O.off, dv, mask8, mask, ra, rb, x, sa, sb);
+        y[ra] = accumulate ? y[ra] + sa : sa;
+        if (rb < n) y[rb] = accumulate ? y[rb] + sb : sb;
     }
 }
 
@@ -101,8 +104,10 @@ Offsets offsets_of(const Mat& A)
 int launch_dia_spmv(const Mat& A, const double* x, double* y, int accumulate)
 {
     if (A.n == 0) return CEL_OK;
-    hipLaunchKernelGGL(k_dia_spmv<double>, dim3(grid_of(A.n)), dim3(kBlock), 0, rt().stream, A.n,
-                       offsets_of(A), A.dia_vals.p, A.dia_mask.p, x, y, accumulate);
+    const long long pairs = ((long long)A.n + 2 * kBlock - 1) / (2 * kBlock);
+    const int G = (int)std::min<long long>(pairs, (long long)rt().num_cus * 8);
+    hipLaunchKernelGGL(k_dia_spmv<double>, dim3(G), dim3(kBlock), 0, rt().stream, A.n,
+                       offsets_of(A), A.dia_vals.p, A.dia_mask.p, A.dia_mask8.p, x, y, accumulate);
     CEL_HIP(hipGetLastError());
     return CEL_OK;
 }
@@ -119,8 +124,11 @@ int launch_dia_spmv_f32(Mat& A, const float* x, float* y, int accumulate)
 {
     if (A.n == 0) return CEL_OK;
     CEL_TRY(dia_prepare_f32(A));
-    hipLaunchKernelGGL(k_dia_spmv<float>, dim3(grid_of(A.n)), dim3(kBlock), 0, rt().stream, A.n,
-                       offsets_of(A), A.dia_vals32.p, A.dia_mask.p, x, y, accumulate);
+    const long long pairs = ((long long)A.n + 2 * kBlock - 1) / (2 * kBlock);
+    const int G = (int)std::min<long long>(pairs, (long long)rt().num_cus * 8);
+    hipLaunchKernelGGL(k_dia_spmv<float>, dim3(G), dim3(kBlock), 0, rt().stream, A.n,
+                       offsets_of(A), A.dia_vals32.p, A.dia_mask.p, A.dia_mask8.p, x, y,
+                       accumulate);
     CEL_HIP(hipGetLastError());
     return CEL_OK;
 }
@@ -179,13 +187,15 @@ extern "C" int cel_gpu_mat_to_dia(cel_gpu_mat* A)
     CEL_HIP(hipStreamSynchronize(st));
     const size_t cnt = (size_t)nd * (size_t)M.n;
     CEL_TRY(M.dia_vals.alloc(cnt));
-    CEL_TRY(M.dia_mask.alloc(M.n));
+    if (nd <= 8) CEL_TRY(M.dia_mask8.alloc(M.n));
+    else CEL_TRY(M.dia_mask.alloc(M.n));
     CEL_HIP(hipMemsetAsync(M.dia_vals.p, 0, sizeof(double) * cnt, st));
     Offsets O{};
     O.nd = nd;
     for (int d = 0; d < nd; ++d) O.off[d] = hoff[d];
     hipLaunchKernelGGL(k_dia_fill, dim3(grid_of(M.n)), dim3(kBlock), 0, st, M.n, M.xadj.p, M.cols.p,
-                       M.vals.p, O, M.dia_vals.p, M.dia_mask.p);
+                       M.vals.p, O, M.dia_vals.p, nd <= 8 ? nullptr : M.dia_mask.p,
+                       nd <= 8 ? M.dia_mask8.p : nullptr);
     CEL_HIP(hipGetLastError());
     CEL_HIP(hipStreamSynchronize(st));
     M.dia_nd = nd;

@@ -124,7 +124,8 @@ struct MatView {
     // DIA: values [nd][n], presence mask, ascending offsets
     const double* dvals;
     const float* dvals32;
-    const uint32_t* dmask;
+    const uint32_t* dmask;   // presence bits, more than 8 diagonals
+    const uint8_t* dmask8;   // presence bits, at most 8 diagonals
     int nd;
     int doff[32];
     // halo (FP64 only)
@@ -154,41 +155,7 @@ template <typename VT>
 __device__ __forceinline__ void dia_rows2(const MatView& A, int ra, int rb,
                                           const VT* __restrict__ x, VT& sa, VT& sb)
 {
-    // buffer resources: one per diagonal (uniform) and one over x; offsets
-    // are 32-bit bytes, and an x offset outside [0, n) (an absent entry of a
-    // boundary row) reads 0 instead of faulting -- it is skipped anyway
-    const int n = A.n;
-    constexpr int B = (int)sizeof(VT);
-    const int rbc = rb < n ? rb : ra;
-    const unsigned ma = A.dmask[ra], mb = A.dmask[rbc];
-    const auto rx = tile_rsrc(x, n * B);
-    sa = VT(0);
-    sb = VT(0);
-    for (int d0 = 0; d0 < A.nd; d0 += 8) {
-        VT va[8], xa[8], vb[8], xb[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int d = d0 + k;
-            if (d < A.nd) {
-                const auto rv = tile_rsrc(dia_vals<VT>(A) + (size_t)d * (size_t)n, n * B);
-                const int o = A.doff[d];
-                va[k] = buf_load_nt<VT>(rv, ra * B, 0);
-                vb[k] = buf_load_nt<VT>(rv, rbc * B, 0);
-                xa[k] = buf_load<VT>(rx, (ra + o) * B);
-                xb[k] = buf_load<VT>(rx, (rbc + o) * B);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int d = d0 + k;
-            if (d < A.nd) {
-                const VT pa = va[k] * xa[k];
-                const VT pb = vb[k] * xb[k];
-                sa = ((ma >> d) & 1u) ? sa + pa : sa;
-                sb = ((mb >> d) & 1u) ? sb + pb : sb;
-            }
-        }
-    }
+    dia_row_pair<VT>(A.n, A.nd, A.doff, dia_vals<VT>(A), A.dmask8, A.dmask, ra, rb, x, sa, sb);
 }
 
 template <typename VT>
@@ -908,6 +875,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     V.dvals = A.dia_vals.p;
     V.dvals32 = A.dia_vals32.p;
     V.dmask = A.dia_mask.p;
+    V.dmask8 = A.dia_mask8.p;
     V.nd = A.dia_nd;
     for (int d = 0; d < A.dia_nd; ++d) V.doff[d] = A.dia_off[d];
     if (halo) {

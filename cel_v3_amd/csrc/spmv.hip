@@ -453,7 +453,7 @@ double spmv_bytes(const Mat& A, int format, int precision)
     const double vb = precision == CEL_PREC_FP64 ? 8.0 : 4.0;
     const double n = A.n, nnz = A.nnz;
     if (format == CEL_FMT_DIA)  // stored slots, the row masks, x once, y
-        return vb * A.dia_nd * n + 4.0 * n + vb * A.ncols + vb * n;
+        return vb * A.dia_nd * n + (A.dia_nd <= 8 ? 1.0 : 4.0) * n + vb * A.ncols + vb * n;
     double idx = (format == CEL_FMT_JAD) ? 4.0 * (A.jad_mmax + 1) + 4.0 * n : 4.0 * (n + 1);
     return nnz * (4.0 + vb) + idx + vb * A.ncols + vb * n;
 }
