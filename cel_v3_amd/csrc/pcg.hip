@@ -9,7 +9,8 @@
 //     beta = (r.z)/alpha ; alpha = r.z ; p = beta p + z ; norm2 = ||r||
 //
 // One iteration is three streaming kernels:
-//   K1  v = A p (LDS-staged CSR tile or JAD), one partial of v.p per block
+//   K1  v = A p (LDS-staged CSR tile, DIA rows or JAD), one partial of v.p
+//       per block
 //   K2  sigma = alpha / sum(v.p) ; r = r + (-sigma) v ; z = jac*r ;
 //       partials of r.z and r.r
 //   K3  beta = sum(r.z)/alpha ; x = x + sigma p ; p = beta p + jac*r ;
@@ -299,6 +300,20 @@ __global__ __launch_bounds__(kBlock) void k_residual(MatView A, const double* __
     write_ctrl(cout, c);
 }
 
+// Tile order of the CSR and DIA K1: step k of a block covers tile
+// 2q + (k & 1) of the tile pair q = blockIdx + (k >> 1) * G (-1: past the
+// end).  Both forms add a thread's rows to the block partial in this order,
+// so their partials and solves are bit-identical; neighbouring blocks
+// stream neighbouring pairs (measured faster on the 256^3 solve than a
+// contiguous tile range per block, for both forms).
+__device__ __forceinline__ long long pair_tile(long long k, long long ntiles)
+{
+    const long long npairs = (ntiles + 1) / 2;
+    const long long q = (long long)blockIdx.x + (k >> 1) * gridDim.x;
+    const long long t = 2 * q + (k & 1);
+    return (q < npairs && t < ntiles) ? t : -1;
+}
+
 // K1: v = A p, per-block partial of v.p.  Folds pending set-up or
 // replacement partials into the token first (every block, same order) and
 // decides whether the iteration runs.
@@ -348,8 +363,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
     __shared__ CsrStage<VT> sm;
     const int lb = logical_block();
     const long long ntiles = (A.n + kTileRows - 1) / kTileRows;
-    long long t0, t1;
-    unit_range(ntiles, gridDim.x, lb, t0, t1);
     double acc = 0.0;
     if constexpr (FMT == CEL_FMT_CSR) {
         // the halo products of each tile are computed into LDS while the
@@ -362,16 +375,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
             halo_stage_init(hs);
             __syncthreads();
         }
-        if (t0 < t1) {  // uniform across the block
-            cs.fetch(A.n, (int)t0, true, A.xadj, A.cols, vals);
+        const long long tfirst = pair_tile(0, ntiles);
+        if (tfirst >= 0) {  // uniform across the block
+            cs.fetch(A.n, (int)tfirst, true, A.xadj, A.cols, vals);
             cs.stage(sm);
             if constexpr (HALO)
-                hstaged = halo_stage_mark(hs, (int)t0, A.hrow_tile, A.hrows, A.hptr, A.hcols,
+                hstaged = halo_stage_mark(hs, (int)tfirst, A.hrow_tile, A.hrows, A.hptr, A.hcols,
                                           A.hvals, A.ghost);
             __syncthreads();
         }
-        for (long long t = t0; t < t1; ++t) {
-            const int nxt = (t + 1 < t1) ? (int)(t + 1) : -1;
+        long long k = 0;
+        for (long long t = tfirst; t >= 0; t = pair_tile(++k, ntiles)) {
+            const int nxt = (int)pair_tile(k + 1, ntiles);
             const int row = (int)t * kTileRows + threadIdx.x;
             VT pr = VT(0);
             VT s = cs.template rows<VT>(A.n, (int)t, nxt, A.xadj, A.cols, vals, p, sm, row, &pr);
@@ -393,11 +408,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
             }
         }
     } else if constexpr (FMT == CEL_FMT_DIA) {
-        // two tiles per step; the partial adds the rows in tile order, as
-        // the CSR K1 does on the same grid
-        for (long long t = t0; t < t1; t += 2) {
-            const int ra = (int)t * kTileRows + threadIdx.x;
-            const int rb = t + 1 < t1 ? ra + kTileRows : A.n;
+        // both tiles of a pair per step (the rows of the first tile are
+        // added to the partial first, as in the CSR K1)
+        const long long npairs = (ntiles + 1) / 2;
+        for (long long q = blockIdx.x; q < npairs; q += gridDim.x) {
+            const int ra = (int)(2 * q) * kTileRows + threadIdx.x;
+            const int rb = 2 * q + 1 < ntiles ? ra + kTileRows : A.n;
             if (ra < A.n) {
                 VT sa, sb;
                 dia_rows2<VT>(A, ra, rb, p, sa, sb);
@@ -416,6 +432,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
             }
         }
     } else {
+        long long t0, t1;
+        unit_range(ntiles, gridDim.x, lb, t0, t1);
         TileWalk<VT, VT, FMT, false> walk;
         walk.start(A, t0, t1, sm);
         for (long long t = t0; t < t1; ++t) {
