@@ -148,17 +148,6 @@ __device__ __forceinline__ const double* dia_vals<double>(const MatView& A) { re
 template <>
 __device__ __forceinline__ const float* dia_vals<float>(const MatView& A) { return A.dvals32; }
 
-// DIA row sums of two rows (rb may be past the end: its result is not
-// used): the row's entries in ascending offset (= CSR column) order, absent
-// ones skipped.  The loads of a group of 8 diagonals of both rows are
-// issued before any of their products is added.
-template <typename VT>
-__device__ __forceinline__ void dia_rows2(const MatView& A, int ra, int rb,
-                                          const VT* __restrict__ x, VT& sa, VT& sb)
-{
-    dia_row_pair<VT>(A.n, A.nd, A.doff, dia_vals<VT>(A), A.dmask8, A.dmask, ra, rb, x, sa, sb);
-}
-
 template <typename VT>
 __device__ __forceinline__ const VT* jad_vals(const MatView& A);
 template <>
@@ -300,18 +289,23 @@ __global__ __launch_bounds__(kBlock) void k_residual(MatView A, const double* __
     write_ctrl(cout, c);
 }
 
-// Tile order of the CSR and DIA K1: step k of a block covers tile
-// 2q + (k & 1) of the tile pair q = blockIdx + (k >> 1) * G (-1: past the
-// end).  Both forms add a thread's rows to the block partial in this order,
-// so their partials and solves are bit-identical; neighbouring blocks
-// stream neighbouring pairs (measured faster on the 256^3 solve than a
-// contiguous tile range per block, for both forms).
-__device__ __forceinline__ long long pair_tile(long long k, long long ntiles)
+// Tile order of the CSR and DIA K1: tiles go in groups of kK1Group
+// consecutive tiles, group q to block q mod G; step k of a block covers
+// tile kK1Group*q + k mod kK1Group of its (k / kK1Group)-th group
+// (-1: past the end).  Both forms add a thread's rows to the block partial
+// in this order, so their partials and solves are bit-identical;
+// neighbouring blocks stream neighbouring groups (measured faster on the
+// 256^3 solve than a contiguous tile range per block, for both forms).
+#ifndef CEL_K1_GROUP
+#define CEL_K1_GROUP 2
+#endif
+constexpr int kK1Group = CEL_K1_GROUP;
+__device__ __forceinline__ long long group_tile(long long k, long long ntiles)
 {
-    const long long npairs = (ntiles + 1) / 2;
-    const long long q = (long long)blockIdx.x + (k >> 1) * gridDim.x;
-    const long long t = 2 * q + (k & 1);
-    return (q < npairs && t < ntiles) ? t : -1;
+    const long long ngroups = (ntiles + kK1Group - 1) / kK1Group;
+    const long long q = (long long)blockIdx.x + (k / kK1Group) * gridDim.x;
+    const long long t = kK1Group * q + k % kK1Group;
+    return (q < ngroups && t < ntiles) ? t : -1;
 }
 
 // K1: v = A p, per-block partial of v.p.  Folds pending set-up or
@@ -375,7 +369,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
             halo_stage_init(hs);
             __syncthreads();
         }
-        const long long tfirst = pair_tile(0, ntiles);
+        const long long tfirst = group_tile(0, ntiles);
         if (tfirst >= 0) {  // uniform across the block
             cs.fetch(A.n, (int)tfirst, true, A.xadj, A.cols, vals);
             cs.stage(sm);
@@ -385,8 +379,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
             __syncthreads();
         }
         long long k = 0;
-        for (long long t = tfirst; t >= 0; t = pair_tile(++k, ntiles)) {
-            const int nxt = (int)pair_tile(k + 1, ntiles);
+        for (long long t = tfirst; t >= 0; t = group_tile(++k, ntiles)) {
+            const int nxt = (int)group_tile(k + 1, ntiles);
             const int row = (int)t * kTileRows + threadIdx.x;
             VT pr = VT(0);
             VT s = cs.template rows<VT>(A.n, (int)t, nxt, A.xadj, A.cols, vals, p, sm, row, &pr);
@@ -408,26 +402,30 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
             }
         }
     } else if constexpr (FMT == CEL_FMT_DIA) {
-        // both tiles of a pair per step (the rows of the first tile are
-        // added to the partial first, as in the CSR K1)
-        const long long npairs = (ntiles + 1) / 2;
-        for (long long q = blockIdx.x; q < npairs; q += gridDim.x) {
-            const int ra = (int)(2 * q) * kTileRows + threadIdx.x;
-            const int rb = 2 * q + 1 < ntiles ? ra + kTileRows : A.n;
-            if (ra < A.n) {
-                VT sa, sb;
-                dia_rows2<VT>(A, ra, rb, p, sa, sb);
-                const VT pa = p[ra];
-                sa = VT(0) + sa;
-                v[ra] = sa;
-                double prod = double(sa) * double(pa);
-                acc = acc + prod;
-                if (rb < A.n) {
-                    const VT pb = p[rb];
-                    sb = VT(0) + sb;
-                    v[rb] = sb;
-                    prod = double(sb) * double(pb);
-                    acc = acc + prod;
+        // all tiles of a group per step, one row of each per thread; the
+        // rows are added to the partial in tile order, as in the CSR K1
+        constexpr int R = kK1Group;
+        const long long ngroups = (ntiles + R - 1) / R;
+        for (long long q = blockIdx.x; q < ngroups; q += gridDim.x) {
+            int r[R];
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const long long t = R * q + k;
+                r[k] = t < ntiles ? (int)t * kTileRows + threadIdx.x : A.n;
+            }
+            if (r[0] < A.n) {
+                VT s[R];
+                dia_rows<VT, R, (R >= 4 ? 4 : 8)>(A.n, A.nd, A.doff, dia_vals<VT>(A), A.dmask8,
+                                                   A.dmask, r, p, s);
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    if (r[k] < A.n) {
+                        const VT pk = p[r[k]];
+                        const VT sk = VT(0) + s[k];
+                        v[r[k]] = sk;
+                        double prod = double(sk) * double(pk);
+                        acc = acc + prod;
+                    }
                 }
             }
         }

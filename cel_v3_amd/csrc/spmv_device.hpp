@@ -97,55 +97,55 @@ __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int v
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
 }
 
-// DIA row sums of two rows ra and rb (rb >= n: not used) of a block stored
-// per diagonal (vals[d*n + i]; offsets ascending; bit d of the row mask marks
-// an entry): the row's entries in ascending offset (= CSR column) order,
-// absent ones skipped, so the sums are the CSR row sums bit for bit.  A
-// group of 8 diagonals of both rows is loaded before any product is added;
-// values through non-temporal buffer loads (one resource per diagonal), x
-// through one buffer resource, so the x of an absent boundary entry reads 0
-// from outside the resource instead of needing a clamp.  n * sizeof(VT)
-// must stay below 2^31 (32-bit byte offsets).
-template <typename VT>
-__device__ __forceinline__ void dia_row_pair(int n, int nd, const int* doff, const VT* vals,
-                                             const uint8_t* m8, const uint32_t* m32, int ra,
-                                             int rb, const VT* __restrict__ x, VT& sa, VT& sb)
+// DIA row sums of R rows r[0..R) (r[k] >= n: not used; r[0] < n) of a block
+// stored per diagonal (vals[d*n + i]; offsets ascending; bit d of the row
+// mask marks an entry): each row's entries in ascending offset (= CSR
+// column) order, absent ones skipped, so the sums are the CSR row sums bit
+// for bit.  A group of GD diagonals of all R rows is loaded before any of
+// their products is added; values through non-temporal buffer loads (one
+// resource per diagonal), x through one buffer resource, so the x of an
+// absent boundary entry reads 0 from outside the resource instead of
+// needing a clamp.  n * sizeof(VT) must stay below 2^31 (32-bit byte
+// offsets).
+template <typename VT, int R, int GD>
+__device__ __forceinline__ void dia_rows(int n, int nd, const int* doff, const VT* vals,
+                                         const uint8_t* m8, const uint32_t* m32, const int (&r)[R],
+                                         const VT* __restrict__ x, VT (&s)[R])
 {
     constexpr int B = (int)sizeof(VT);
-    const int rbc = rb < n ? rb : ra;
-    unsigned ma, mb;
-    if (m8) {  // uniform
-        ma = m8[ra];
-        mb = m8[rbc];
-    } else {
-        ma = m32[ra];
-        mb = m32[rbc];
+    int rc[R];
+    unsigned m[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        rc[k] = r[k] < n ? r[k] : r[0];
+        m[k] = m8 ? (unsigned)m8[rc[k]] : m32[rc[k]];  // m8: uniform
+        s[k] = VT(0);
     }
     const auto rx = tile_rsrc(x, n * B);
-    sa = VT(0);
-    sb = VT(0);
-    for (int d0 = 0; d0 < nd; d0 += 8) {
-        VT va[8], xa[8], vb[8], xb[8];
+    for (int d0 = 0; d0 < nd; d0 += GD) {
+        VT v[GD][R], xv[GD][R];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int d = d0 + k;
+        for (int g = 0; g < GD; ++g) {
+            const int d = d0 + g;
             if (d < nd) {
                 const auto rv = tile_rsrc(vals + (size_t)d * (size_t)n, n * B);
                 const int o = doff[d];
-                va[k] = buf_load_nt<VT>(rv, ra * B, 0);
-                vb[k] = buf_load_nt<VT>(rv, rbc * B, 0);
-                xa[k] = buf_load<VT>(rx, (ra + o) * B);
-                xb[k] = buf_load<VT>(rx, (rbc + o) * B);
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    v[g][k] = buf_load_nt<VT>(rv, rc[k] * B, 0);
+                    xv[g][k] = buf_load<VT>(rx, (rc[k] + o) * B);
+                }
             }
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int d = d0 + k;
+        for (int g = 0; g < GD; ++g) {
+            const int d = d0 + g;
             if (d < nd) {
-                const VT pa = va[k] * xa[k];
-                const VT pb = vb[k] * xb[k];
-                sa = ((ma >> d) & 1u) ? sa + pa : sa;
-                sb = ((mb >> d) & 1u) ? sb + pb : sb;
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const VT prod = v[g][k] * xv[g][k];
+                    s[k] = ((m[k] >> d) & 1u) ? s[k] + prod : s[k];
+                }
             }
         }
     }
