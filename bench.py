@@ -334,8 +334,7 @@ def main():
     log(f"n={M.n} nnz={M.nnz} |b|={bnorm:.6e}")
 
     def solve(precision, profile=0, halo=0, form=None):
-        # the halo blocks are applied by the CSR K1
-        f = cel.FMT_CSR if halo else (fmt if form is None else form)
+        f = fmt if form is None else form
         r = M.pcg(d_b, x0=None, tol=args.tol, tol_mode=cel.TOL_REL_B, x0_mode=cel.X0_ZERO,
                   iter_max=20000, precision=precision, fmt=f, profile=profile, use_halo=halo,
                   allow_not_converged=False)
@@ -359,10 +358,11 @@ def main():
         ms = cel.stream_copy_time(copy_bytes, reps=2)
         prof = {"copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes,
                 "copy_ms": ms}
-        runs = [("fp64", cel.PREC_FP64, 0, fmt), ("fp64_halo", cel.PREC_FP64, use_halo, cel.FMT_CSR),
+        runs = [("fp64", cel.PREC_FP64, 0, fmt), ("fp64_halo", cel.PREC_FP64, use_halo, fmt),
                 ("mixed", cel.PREC_MIXED, 0, fmt)]
         if fmt == cel.FMT_DIA:
-            runs += [("fp64_csr", cel.PREC_FP64, 0, cel.FMT_CSR), ("mixed_csr", cel.PREC_MIXED, 0, cel.FMT_CSR)]
+            runs += [("fp64_csr", cel.PREC_FP64, 0, cel.FMT_CSR), ("mixed_csr", cel.PREC_MIXED, 0, cel.FMT_CSR),
+                     ("fp64_halo_csr", cel.PREC_FP64, use_halo, cel.FMT_CSR)]
         for name, prec, halo, f in runs:
             r = M.pcg(d_b, x0=None, tol=args.tol, tol_mode=cel.TOL_REL_B, x0_mode=cel.X0_ZERO,
                       iter_max=args.iters, precision=prec, fmt=f, use_halo=halo, profile=1)
@@ -464,7 +464,12 @@ def main():
         if fmt == cel.FMT_DIA:
             prof["fp64_csr"] = solve(cel.PREC_FP64, profile=1, form=cel.FMT_CSR)
         if use_halo:
+            halo_timed = {"dia" if fmt == cel.FMT_DIA else "csr": solve(cel.PREC_FP64, halo=1)}
             prof["fp64_halo"] = solve(cel.PREC_FP64, profile=1, halo=1)
+            if fmt == cel.FMT_DIA:
+                halo_timed["csr"] = solve(cel.PREC_FP64, halo=1, form=cel.FMT_CSR)
+                prof["fp64_halo_csr"] = solve(cel.PREC_FP64, profile=1, halo=1, form=cel.FMT_CSR)
+            log("fp64 with halo faces: " + ", ".join(f"{k} {v['solve_ms']:.1f} ms" for k, v in halo_timed.items()))
 
     def k1(r):
         ms = r["spmv_ms"] / max(r["spmv_launches"], 1)
@@ -526,6 +531,8 @@ def main():
                             ("roofline_csr", "mixed_csr", "k_pcg_spmv<float,CSR> (v = A p fused with v.p)"),
                             ("roofline_csr_fp64", "fp64_csr", "k_pcg_spmv<double,CSR> (v = A p fused with v.p)"),
                             ("roofline_fp64_halo", "fp64_halo",
+                             f"k_pcg_spmv<double,{fname},halo> (+6 halo faces, one-subdomain model)"),
+                            ("roofline_csr_fp64_halo", "fp64_halo_csr",
                              "k_pcg_spmv<double,CSR,halo> (+6 halo faces, one-subdomain model)")):
         if form not in prof:
             continue
@@ -540,10 +547,16 @@ def main():
             f"({achieved / HBM_PEAK_GBS:.1%} of 8 TB/s)")
     if "fp64_halo" in prof:
         r = prof["fp64_halo"]
-        res["fp64_halo"] = {"solve_ms": r["solve_ms"], "iters": r["iters"],
+        first = next(iter(halo_timed.values()))
+        res["fp64_halo"] = {"solve_ms": first["solve_ms"], "iters": first["iters"], "format": args.format,
                             "halo_faces": 6, "face_entries": 6 * N * N,
                             "note": "one-subdomain model: 6 halo blocks and face packing every "
                                     "SpMV, received values zero"}
+        if "csr" in halo_timed and fmt == cel.FMT_DIA:
+            rc = halo_timed["csr"]
+            res["fp64_halo"]["csr_solve_ms"] = rc["solve_ms"]
+            res["fp64_halo"]["solution_bitwise_equal_to_csr"] = bool(
+                np.array_equal(first["x"].to_host(), rc["x"].to_host()))
     res["config"] = cfg
     res.update(out)
     if not args.skip_cpu and fp64_runs and args.only == "all":

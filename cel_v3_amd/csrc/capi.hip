@@ -634,6 +634,7 @@ static int rebuild_halo(Mat& M)
     }
     M.halo_nrows = (int)hrows.size();
     CEL_TRY(upload(M.halo_rows, hrows.data(), hrows.size()));
+    CEL_TRY(upload(M.halo_rowq, rowq.data(), rowq.size()));
     CEL_TRY(upload(M.halo_ptr, hptr.data(), hptr.size()));
     CEL_TRY(upload(M.halo_cols, hcols.data(), hcols.size()));
     CEL_TRY(upload(M.halo_vals, hvals.data(), hvals.size()));

@@ -132,6 +132,7 @@ struct Mat {
     int ghost_len = 0;             // sum of recv_len over halo blocks
     DevBuf<int> halo_xadj;         // CSR over local rows of the merged halo entries (n+1)
     DevBuf<int> halo_rows;         // rows having halo entries (compact list)
+    DevBuf<int> halo_rowq;         // per local row: its compact index, or -1
     DevBuf<int> halo_ptr;          // per compact row: start into halo_cols/vals (+1 end)
     DevBuf<int> halo_cols;         // ghost slot per entry, in block order
     DevBuf<double> halo_vals;
@@ -170,6 +171,9 @@ int launch_jad_spmv_f32(const Mat& A, const float* x, float* y, int accumulate);
 int launch_dia_spmv(const Mat& A, const double* x, double* y, int accumulate);
 int launch_dia_spmv_f32(Mat& A, const float* x, float* y, int accumulate);
 int dia_prepare_f32(Mat& A);
+// Sets bit `bit` (= dia_nd) of the DIA row masks on the rows with halo
+// entries and clears it elsewhere; fails when the masks have no spare bit.
+int dia_mark_halo_rows(Mat& A, int& bit);
 int launch_halo_spmv(const Mat& A, const double* ghost, double* y);
 int launch_face_pack(const Mat& A, const double* x, double* sendbuf);
 int launch_extract_diag(Mat& A);

@@ -128,9 +128,10 @@ struct MatView {
     const uint32_t* dmask;   // presence bits, more than 8 diagonals
     const uint8_t* dmask8;   // presence bits, at most 8 diagonals
     int nd;
+    int hbit;                // with halo: mask bit of the rows that have halo entries
     int doff[32];
     // halo (FP64 only)
-    const int *hrow_tile, *hrows, *hptr, *hcols;
+    const int *hrow_tile, *hrows, *hrowq, *hptr, *hcols;
     const double* hvals;
     const double* ghost;
 };
@@ -421,7 +422,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
                 for (int k = 0; k < R; ++k) {
                     if (r[k] < A.n) {
                         const VT pk = p[r[k]];
-                        const VT sk = VT(0) + s[k];
+                        VT sk = VT(0) + s[k];
+                        if constexpr (HALO) {
+                            // rows flagged in the masks add their halo
+                            // products after the local sum, in block order,
+                            // as the CSR K1 does
+                            const unsigned mk = A.dmask8 ? (unsigned)A.dmask8[r[k]] : A.dmask[r[k]];
+                            if ((mk >> A.hbit) & 1u) {
+                                const int hq = A.hrowq[r[k]];
+                                for (int j = A.hptr[hq]; j < A.hptr[hq + 1]; ++j) {
+                                    VT prod = VT(A.hvals[j]) * VT(A.ghost[A.hcols[j]]);
+                                    sk = sk + prod;
+                                }
+                            }
+                        }
                         v[r[k]] = sk;
                         double prod = double(sk) * double(pk);
                         acc = acc + prod;
@@ -748,8 +762,8 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         return fail(CEL_ERR_STATE, "JAD requested but the matrix has no complete JAD arrays");
     if (fmt == CEL_FMT_DIA && A.dia_nd == 0)
         return fail(CEL_ERR_STATE, "DIA requested but the matrix has no DIA form");
-    if (halo && (mixed || fmt != CEL_FMT_CSR))
-        return fail(CEL_ERR_ARG, "halo blocks are applied by the FP64 CSR form only");
+    if (halo && (mixed || fmt == CEL_FMT_JAD))
+        return fail(CEL_ERR_ARG, "halo blocks are applied by the FP64 CSR and DIA forms only");
     if (!A.has_jac) return fail(CEL_ERR_STATE, "cel_gpu_mat_set_jacobi has not been called");
     if (A.ncols != n) return fail(CEL_ERR_ARG, "PCG needs a square local block");
     if (prm.iter_max < 0) return fail(CEL_ERR_ARG, "iter_max < 0");
@@ -768,15 +782,11 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     // same partial sums); the residual kernels of a DIA solve use the CSR form
     const int rfmt = fmt == CEL_FMT_DIA ? CEL_FMT_CSR : fmt;
     if (!mixed) {
-        if (fmt == CEL_FMT_DIA) {
+        if (fmt == CEL_FMT_DIA || fmt == CEL_FMT_CSR) {
             G1 = common_grid({(const void*)k_pcg_spmv<double, CEL_FMT_CSR, false>,
                               (const void*)k_pcg_spmv<double, CEL_FMT_CSR, true>,
-                              (const void*)k_pcg_spmv<double, CEL_FMT_DIA, false>}, ntiles);
-            Gr = common_grid({(const void*)k_residual<double, CEL_FMT_CSR, false, false>,
-                              (const void*)k_residual<double, CEL_FMT_CSR, true, false>}, ntiles);
-        } else if (fmt == CEL_FMT_CSR) {
-            G1 = common_grid({(const void*)k_pcg_spmv<double, CEL_FMT_CSR, false>,
-                              (const void*)k_pcg_spmv<double, CEL_FMT_CSR, true>}, ntiles);
+                              (const void*)k_pcg_spmv<double, CEL_FMT_DIA, false>,
+                              (const void*)k_pcg_spmv<double, CEL_FMT_DIA, true>}, ntiles);
             Gr = common_grid({(const void*)k_residual<double, CEL_FMT_CSR, false, false>,
                               (const void*)k_residual<double, CEL_FMT_CSR, true, false>}, ntiles);
         } else {
@@ -784,13 +794,9 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
             Gr = common_grid({(const void*)k_residual<double, CEL_FMT_JAD, false, false>}, ntiles);
         }
     } else {
-        if (fmt == CEL_FMT_DIA) {
+        if (fmt == CEL_FMT_DIA || fmt == CEL_FMT_CSR) {
             G1 = common_grid({(const void*)k_pcg_spmv<float, CEL_FMT_CSR, false>,
                               (const void*)k_pcg_spmv<float, CEL_FMT_DIA, false>}, ntiles);
-            Gr = common_grid({(const void*)k_residual<float, CEL_FMT_CSR, false, false>,
-                              (const void*)k_residual<float, CEL_FMT_CSR, false, true>}, ntiles);
-        } else if (fmt == CEL_FMT_CSR) {
-            G1 = common_grid({(const void*)k_pcg_spmv<float, CEL_FMT_CSR, false>}, ntiles);
             Gr = common_grid({(const void*)k_residual<float, CEL_FMT_CSR, false, false>,
                               (const void*)k_residual<float, CEL_FMT_CSR, false, true>}, ntiles);
         } else {
@@ -854,6 +860,8 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         hipLaunchKernelGGL(k_fill, dim3(64), dim3(kBlock), 0, s, (long long)W.ghost.n, W.ghost.p, 0.0);
         CEL_TRY(build_hrow_tile(A, W, ntiles));
     }
+    int hbit = 0;  // DIA with halo: mask bit that flags the rows with halo entries
+    if (halo && fmt == CEL_FMT_DIA) CEL_TRY(dia_mark_halo_rows(A, hbit));
 
     Prm P{};
     P.tol = prm.tol;
@@ -893,10 +901,12 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     V.dmask = A.dia_mask.p;
     V.dmask8 = A.dia_mask8.p;
     V.nd = A.dia_nd;
+    V.hbit = hbit;
     for (int d = 0; d < A.dia_nd; ++d) V.doff[d] = A.dia_off[d];
     if (halo) {
         V.hrow_tile = W.hrow_tile.p;
         V.hrows = A.halo_rows.p;
+        V.hrowq = A.halo_rowq.p;
         V.hptr = A.halo_ptr.p;
         V.hcols = A.halo_cols.p;
         V.hvals = A.halo_vals.p;
@@ -964,7 +974,10 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
             CEL_HIP(hipEventRecord(ea, s));
         }
         if (!mixed) {
-            if (halo)
+            if (halo && fmt == CEL_FMT_DIA)
+                hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_DIA, true>), dim3(G1), dim3(kBlock), 0, s, V,
+                                   W.p.p, W.v.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
+            else if (halo)
                 hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_CSR, true>), dim3(G1), dim3(kBlock), 0, s, V,
                                    W.p.p, W.v.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
             else if (fmt == CEL_FMT_CSR)

@@ -104,6 +104,41 @@ def test_pcg_with_dia_is_bitwise_the_csr_solve(gpu, precision):
     assert a["final_norm"] == d["final_norm"]
 
 
+def test_pcg_dia_with_halo_faces_is_bitwise_the_csr_solve(gpu):
+    """The centre block of a 3x3x3 split (six halo faces, face packing every
+    iteration): the DIA K1 flags the rows with halo entries in a spare bit
+    of the row masks and adds their halo products after the local sum, in
+    block order, as the CSR K1 does.  The two halo solves, and the local
+    solve (the ghost region of one subdomain is zero), agree bit for bit;
+    the flag bit leaves the plain DIA product and solve unchanged."""
+    from test_gpu_distr import block_major_poisson
+
+    m, d, rank = 12, 3, 13
+    vtx, rows, cols, vals = block_major_poisson(m, d, rank)
+    M = gpu.Matrix.distr_split(vtx, rank, rows, cols, vals)
+    for blk_id in range(1, 7):
+        blk = M.get_block(blk_id)
+        M.set_send_list(blk["proc"], np.unique(blk["rows"] - vtx[rank] + 1))
+    M.set_jacobi()
+    assert len(M.to_dia()) == 7
+    rng = np.random.default_rng(67)
+    b = rng.random(M.n)
+    kw = dict(tol=1e-10, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, iter_max=2000, history=True)
+    c = M.pcg(b, use_halo=1, **kw)
+    h = M.pcg(b, use_halo=1, fmt=gpu.FMT_DIA, **kw)
+    loc = M.pcg(b, fmt=gpu.FMT_DIA, **kw)
+    assert c["status"] == h["status"] == loc["status"] == 0
+    assert c["iters"] == h["iters"] == loc["iters"]
+    np.testing.assert_array_equal(c["hist"], h["hist"])
+    np.testing.assert_array_equal(c["x"], h["x"])
+    np.testing.assert_array_equal(h["x"], loc["x"])
+    x = rng.normal(size=M.n)
+    np.testing.assert_array_equal(M.spmv(x, fmt=gpu.FMT_DIA), M.spmv(x))
+    # the halo path is FP64 (CSR or DIA form)
+    with pytest.raises(gpu.CelError):
+        M.pcg(b, use_halo=1, fmt=gpu.FMT_DIA, precision=gpu.PREC_MIXED, **kw)
+
+
 def test_pcg_dia_256_timing(gpu):
     """The 256^3 bench block: CSR and DIA mixed solves agree bit for bit;
     prints both times and the K1 means."""
