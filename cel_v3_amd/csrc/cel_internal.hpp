@@ -164,11 +164,17 @@ namespace cel {
 // All launches go to rt().stream.
 int launch_gen_poisson7(int nx, int ny, int nz, const double* kappa, int* xadj, int* cols,
                         double* vals);
-int launch_csr_spmv(const Mat& A, const double* x, double* y, int accumulate);
+// skip (may be null): the launch returns at once when *skip is non-zero on
+// the device (for launches captured in a graph whose later replays must do
+// nothing)
+int launch_csr_spmv(const Mat& A, const double* x, double* y, int accumulate,
+                    const int* skip = nullptr);
 int launch_csr_spmv_f32(const Mat& A, const float* x, float* y, int accumulate);
-int launch_jad_spmv(const Mat& A, const double* x, double* y, int accumulate);
+int launch_jad_spmv(const Mat& A, const double* x, double* y, int accumulate,
+                    const int* skip = nullptr);
 int launch_jad_spmv_f32(const Mat& A, const float* x, float* y, int accumulate);
-int launch_dia_spmv(const Mat& A, const double* x, double* y, int accumulate);
+int launch_dia_spmv(const Mat& A, const double* x, double* y, int accumulate,
+                    const int* skip = nullptr);
 int launch_dia_spmv_f32(Mat& A, const float* x, float* y, int accumulate);
 int dia_prepare_f32(Mat& A);
 // Sets bit `bit` (= dia_nd) of the DIA row masks on the rows with halo

@@ -86,8 +86,9 @@ template <typename VT, int R, int GD>
 __global__ void __launch_bounds__(kBlock)
 k_dia_spmv(int n, Offsets O, const VT* __restrict__ dv, const uint32_t* __restrict__ mask,
            const uint8_t* __restrict__ mask8, const VT* __restrict__ x, VT* __restrict__ y,
-           int accumulate)
+           int accumulate, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const long long groups = ((long long)n + R * kBlock - 1) / (R * kBlock);
     for (long long T = blockIdx.x; T < groups; T += gridDim.x) {
         int r[R];
@@ -119,13 +120,13 @@ struct DiaShape<double> {
 };
 
 template <typename VT>
-int launch_dia(const Mat& A, const VT* vals, const VT* x, VT* y, int accumulate)
+int launch_dia(const Mat& A, const VT* vals, const VT* x, VT* y, int accumulate, const int* skip)
 {
     using S = DiaShape<VT>;
     const long long groups = ((long long)A.n + S::R * kBlock - 1) / (S::R * kBlock);
     const int G = (int)std::min<long long>(groups, (long long)rt().num_cus * S::per_cu);
     hipLaunchKernelGGL((k_dia_spmv<VT, S::R, S::GD>), dim3(G), dim3(kBlock), 0, rt().stream, A.n,
-                       offsets_of(A), vals, A.dia_mask.p, A.dia_mask8.p, x, y, accumulate);
+                       offsets_of(A), vals, A.dia_mask.p, A.dia_mask8.p, x, y, accumulate, skip);
     CEL_HIP(hipGetLastError());
     return CEL_OK;
 }
@@ -173,10 +174,10 @@ int dia_mark_halo_rows(Mat& A, int& bit)
     return CEL_OK;
 }
 
-int launch_dia_spmv(const Mat& A, const double* x, double* y, int accumulate)
+int launch_dia_spmv(const Mat& A, const double* x, double* y, int accumulate, const int* skip)
 {
     if (A.n == 0) return CEL_OK;
-    return launch_dia<double>(A, A.dia_vals.p, x, y, accumulate);
+    return launch_dia<double>(A, A.dia_vals.p, x, y, accumulate, skip);
 }
 
 int dia_prepare_f32(Mat& A)
@@ -191,7 +192,7 @@ int launch_dia_spmv_f32(Mat& A, const float* x, float* y, int accumulate)
 {
     if (A.n == 0) return CEL_OK;
     CEL_TRY(dia_prepare_f32(A));
-    return launch_dia<float>(A, A.dia_vals32.p, x, y, accumulate);
+    return launch_dia<float>(A, A.dia_vals32.p, x, y, accumulate, nullptr);
 }
 
 }  // namespace cel

@@ -16,35 +16,72 @@
 //       if not stopped: x += sum alpha_i z_i (back substitution on m)
 //       r = b - A x ; beta = ||r||
 // Element-wise steps round every product before the add, as the reference's
-// loops do; the vector kernels are single passes over HBM with the
-// Gram-Schmidt update of w fused with the next dot product (v_i+1 . w, or
-// ||w|| after the last) and z_j formed where v_j is.  Dot products are
-// per-workgroup partial sums over fixed contiguous chunks, added on the host
-// in workgroup order: the result does not depend on timing, only on n.  The
-// small Hessenberg algebra runs on the host in the reference's order.
+// loops do.
+//
+// Everything stays on the device; the host only enqueues.  The scalars (the
+// Hessenberg columns, rotations, gamma, back-substitution coefficients,
+// residual norms, counters and stop flags) live in one state record in HBM.
+// Each vector kernel writes per-block partial sums over fixed contiguous
+// chunks; the next kernel sums them in a fixed order (every block that needs
+// the value, or a one-block kernel that updates the state), so the result
+// depends on n only, never on timing.  The small Hessenberg algebra
+// (rotations, convergence test, back substitution) runs in one thread of a
+// one-block kernel, in the reference's order.  Partials alternate between
+// two buffers, so no kernel writes the partials it reads, and every state
+// field a kernel writes is one that no other block of that kernel reads.
+//
+// One outer iteration (its residual and m+1 cycles) is captured once as a
+// hipGraph and replayed up to iter_max+1 times; after a replay the host
+// reads the stop flag (one replay in flight).  Once the solve has converged
+// every kernel of the rest of the graph returns at once: the kernels of
+// later columns and cycles test `halt` (set at convergence), the cycle's
+// closing residual and everything after it test `stop` (set after that
+// residual).  v_m+1 is not formed (nothing reads it).
+#include <cmath>
+#include <string>
+#include <utility>
+
 #include "cel_internal.hpp"
+#include "device_common.hpp"
 
 namespace cel {
 namespace {
 
 constexpr int kThreads = 256;
 constexpr int kMaxRestart = 32;
+constexpr int kM1 = kMaxRestart + 2;  // 1-based index range of the small arrays
 
-struct Coefs {
-    double a[kMaxRestart];
+struct GmState {
+    double H[kM1 * kM1];  // H(i, j) = H[i * kM1 + j], 1-based, rotated in place
+    double cs[kM1], sn[kM1], gam[kM1], alp[kM1];
+    double residuum, local_residuum, norm0, eps;
+    int iter, cycles, ncols;  // outer iterations, cycles, Arnoldi columns run
+    int conv_col;             // column at which the current cycle converged (0: none)
+    int nalp;                 // coefficients in alp for the pending x update
+    int halt;                 // converged or stopped: later columns and cycles do nothing
+    int stop;                 // nothing runs any more
+    int status;               // 0 or CEL_ERR_NONFINITE
 };
 
-// Per-workgroup partial of sum(a .* b) over [blk*chunk, min(n, (blk+1)*chunk)).
-__device__ inline void block_partial(double acc, double* __restrict__ part)
+__device__ __forceinline__ double& Hij(GmState* st, int i, int j) { return st->H[i * kM1 + j]; }
+
+// Per-block partial of a sum, written to part[blockIdx].
+__device__ __forceinline__ void put_partial(double acc, double* __restrict__ part, double* red)
 {
-    __shared__ double red[kThreads];
-    red[threadIdx.x] = acc;
+    const double s = block_sum(acc, red);  // valid in thread 0
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Sum of the G partials in a fixed order, returned to every thread.
+__device__ __forceinline__ double fold(const double* __restrict__ part, int G, double* red,
+                                       double* bcast)
+{
+    const double s = sum_partials(part, G, red);
+    if (threadIdx.x == 0) *bcast = s;
     __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+    const double v = *bcast;
+    __syncthreads();
+    return v;
 }
 
 #define CHUNK_LOOP(i)                                                        \
@@ -58,11 +95,22 @@ k_fill(long long n, long long chunk, double* __restrict__ x, double v)
     CHUNK_LOOP(i) x[i] = v;
 }
 
+__global__ void k_state_init(GmState* st, double eps)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        GmState z{};
+        z.eps = eps;
+        *st = z;
+    }
+}
+
 // r = b - r ; partial r . r
 __global__ void __launch_bounds__(kThreads)
 k_resid(long long n, long long chunk, const double* __restrict__ b, double* __restrict__ r,
-        double* __restrict__ part)
+        double* __restrict__ part, const GmState* __restrict__ st)
 {
+    __shared__ double red[kWaves];
+    if (st->stop) return;
     double acc = 0.0;
     CHUNK_LOOP(i) {
         const double ri = b[i] - r[i];
@@ -70,51 +118,92 @@ k_resid(long long n, long long chunk, const double* __restrict__ b, double* __re
         const double p = ri * ri;
         acc = acc + p;
     }
-    block_partial(acc, part);
+    put_partial(acc, part, red);
 }
 
-// v = r * tmp ; z = v * jac
+// Start of an outer iteration (one block): iter += 1 ; residuum = ||r||
+// (the first one is norm0) ; a non-finite residual ends the solve.
 __global__ void __launch_bounds__(kThreads)
-k_scale_jac(long long n, long long chunk, const double* __restrict__ r, double tmp,
-            const double* __restrict__ jac, double* __restrict__ v, double* __restrict__ z)
+k_outer_start(const double* __restrict__ part, int G, GmState* st)
 {
+    __shared__ double red[kWaves];
+    __shared__ double bcast;
+    if (st->stop) return;
+    const double s = fold(part, G, red, &bcast);
+    if (threadIdx.x == 0) {
+        const double res = sqrt(s);
+        st->iter = st->iter + 1;
+        if (st->iter == 1) st->norm0 = res;
+        st->residuum = res;
+        st->local_residuum = res;
+        if (!isfinite(res)) {
+            st->status = CEL_ERR_NONFINITE;
+            st->halt = 1;
+            st->stop = 1;
+        }
+    }
+}
+
+// End of a cycle (one block): residuum = ||r|| ; cycles += 1 ; a cycle that
+// converged ends the solve here.
+__global__ void __launch_bounds__(kThreads)
+k_cycle_end(const double* __restrict__ part, int G, GmState* st)
+{
+    __shared__ double red[kWaves];
+    __shared__ double bcast;
+    if (st->stop) return;
+    const double s = fold(part, G, red, &bcast);
+    if (threadIdx.x == 0) {
+        const double res = sqrt(s);
+        st->residuum = res;
+        st->local_residuum = res;
+        st->cycles = st->cycles + 1;
+        if (st->conv_col) st->stop = 1;
+    }
+}
+
+// v1 = r * (1/beta) ; z1 = v1 * jac ; gamma1 = beta
+__global__ void __launch_bounds__(kThreads)
+k_scale_jac(long long n, long long chunk, const double* __restrict__ r,
+            const double* __restrict__ jac, double* __restrict__ v, double* __restrict__ z,
+            GmState* st)
+{
+    if (st->halt) return;
+    const double tmp = 1.0 / st->local_residuum;
     CHUNK_LOOP(i) {
         const double vi = r[i] * tmp;
         v[i] = vi;
         z[i] = vi * jac[i];
     }
-}
-
-// v = w / h ; z = v * jac
-__global__ void __launch_bounds__(kThreads)
-k_div_jac(long long n, long long chunk, const double* __restrict__ w, double h,
-          const double* __restrict__ jac, double* __restrict__ v, double* z)
-{
-    CHUNK_LOOP(i) {
-        const double vi = w[i] / h;
-        v[i] = vi;
-        if (z) z[i] = vi * jac[i];
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->gam[1] = st->residuum;
 }
 
 // partial a . b
 __global__ void __launch_bounds__(kThreads)
 k_dot(long long n, long long chunk, const double* __restrict__ a, const double* __restrict__ b,
-      double* __restrict__ part)
+      double* __restrict__ part, const GmState* __restrict__ st)
 {
+    __shared__ double red[kWaves];
+    if (st->halt) return;
     double acc = 0.0;
     CHUNK_LOOP(i) {
         const double p = a[i] * b[i];
         acc = acc + p;
     }
-    block_partial(acc, part);
+    put_partial(acc, part, red);
 }
 
-// w = w - h vi ; partial (next ? next . w : w . w)
+// h_ii,jj = sum(prev partials) ; w = w - h vi ; partial (next ? next . w : w . w)
 __global__ void __launch_bounds__(kThreads)
-k_mgs(long long n, long long chunk, double h, const double* __restrict__ vi,
-      double* __restrict__ w, const double* __restrict__ next, double* __restrict__ part)
+k_mgs(long long n, long long chunk, int ii, int jj, const double* __restrict__ prev, int G,
+      const double* __restrict__ vi, double* __restrict__ w, const double* __restrict__ next,
+      double* __restrict__ part, GmState* st)
 {
+    __shared__ double red[kWaves];
+    __shared__ double bcast;
+    if (st->halt) return;
+    const double h = fold(prev, G, red, &bcast);
+    if (blockIdx.x == 0 && threadIdx.x == 0) Hij(st, ii, jj) = h;
     double acc = 0.0;
     CHUNK_LOOP(i) {
         const double t = h * vi[i];
@@ -123,21 +212,97 @@ k_mgs(long long n, long long chunk, double h, const double* __restrict__ vi,
         const double p = (next ? next[i] : wi) * wi;
         acc = acc + p;
     }
-    block_partial(acc, part);
+    put_partial(acc, part, red);
 }
 
-// x = x + a_1 z_1 + ... + a_k z_k, added in that order
-__global__ void __launch_bounds__(kThreads)
-k_xupdate(long long n, long long chunk, int k, Coefs c, const double* __restrict__ Z,
-          long long ld, double* __restrict__ x)
+// Back substitution on the first k columns (cel_gmresalg_module.f90:760-772
+// order), one thread.
+__device__ void back_substitute(GmState* st, int k)
 {
-    CHUNK_LOOP(i) {
-        double xi = x[i];
-        for (int j = 0; j < k; ++j) {
-            const double t = c.a[j] * Z[(long long)j * ld + i];
-            xi = xi + t;
+    for (int ii = k; ii >= 1; --ii) {
+        double a = 0.0;
+        for (int kk = ii + 1; kk <= k; ++kk) {
+            const double t = Hij(st, ii, kk) * st->alp[kk];
+            a = a + t;
         }
-        x[i] = xi;
+        const double inv = 1.0 / Hij(st, ii, ii);
+        st->alp[ii] = inv * (st->gam[ii] - a);
+    }
+    st->nalp = k;
+}
+
+// One block: h_jj+1,jj = sqrt(sum of the last partials) ; the column's
+// rotations as the reference writes them ; new rotation ; gamma ;
+// convergence test ; the back-substitution coefficients when the column
+// converged or is the cycle's last.
+__global__ void __launch_bounds__(kThreads)
+k_givens(int jj, int m, const double* __restrict__ part, int G, GmState* st)
+{
+    __shared__ double red[kWaves];
+    __shared__ double bcast;
+    if (st->halt) return;
+    const double s = fold(part, G, red, &bcast);
+    if (threadIdx.x != 0) return;
+    Hij(st, jj + 1, jj) = sqrt(s);
+    for (int ii = 1; ii <= jj - 1; ++ii) {  // both statements assign h(ii,jj) (:722-727)
+        double a = Hij(st, ii, jj) * st->cs[ii];
+        double c = Hij(st, ii + 1, jj) * st->sn[ii];
+        Hij(st, ii, jj) = a + c;
+        a = -Hij(st, ii, jj) * st->sn[ii];
+        c = Hij(st, ii + 1, jj) * st->cs[ii];
+        Hij(st, ii, jj) = a + c;
+    }
+    {
+        const double a = Hij(st, jj, jj) * Hij(st, jj, jj);
+        const double c = Hij(st, jj + 1, jj) * Hij(st, jj + 1, jj);
+        const double betta = sqrt(a + c);
+        st->sn[jj] = Hij(st, jj + 1, jj) / betta;
+        st->cs[jj] = Hij(st, jj, jj) / betta;
+        Hij(st, jj, jj) = betta;
+        st->gam[jj + 1] = -st->sn[jj] * st->gam[jj];
+        st->gam[jj] = st->cs[jj] * st->gam[jj];
+    }
+    st->ncols = st->ncols + 1;
+    st->nalp = 0;
+    if (fabs(st->gam[jj + 1]) <= st->eps) {
+        back_substitute(st, jj);
+        st->conv_col = jj;
+        st->halt = 1;
+    } else if (jj == m) {
+        back_substitute(st, m);
+    }
+}
+
+// After column jj: with coefficients pending (the column converged or is the
+// cycle's last) x = x + a_1 z_1 + ... + a_k z_k, added in that order; else
+// v_jj+1 = w / h_jj+1,jj and z_jj+1 = v_jj+1 * jac.
+__global__ void __launch_bounds__(kThreads)
+k_next(long long n, long long chunk, int jj, int m, const double* __restrict__ w,
+       const double* __restrict__ jac, double* __restrict__ V, double* __restrict__ Z,
+       double* __restrict__ x, const GmState* __restrict__ st)
+{
+    if (st->stop) return;
+    if (st->halt && st->conv_col != jj) return;  // converged at an earlier column
+    const int k = st->nalp;
+    if (k > 0) {
+        CHUNK_LOOP(i) {
+            double xi = x[i];
+            for (int j = 0; j < k; ++j) {
+                const double t = st->alp[j + 1] * Z[(long long)j * n + i];
+                xi = xi + t;
+            }
+            x[i] = xi;
+        }
+        return;
+    }
+    if (jj == m) return;
+    const double h = st->H[(jj + 1) * kM1 + jj];
+    double* v = V + (long long)jj * n;  // v_jj+1
+    double* z = Z + (long long)jj * n;  // z_jj+1
+    CHUNK_LOOP(i) {
+        const double vi = w[i] / h;
+        v[i] = vi;
+        z[i] = vi * jac[i];
     }
 }
 
@@ -147,9 +312,6 @@ struct Gmres {
     long long n, chunk;
     int G;
     hipStream_t st;
-    DevBuf<double> part;
-    double* hpart = nullptr;
-    int launches = 0;
 
     Gmres(const Mat& M, int fmt) : A(M), format(fmt), n(M.n), st(rt().stream)
     {
@@ -160,43 +322,11 @@ struct Gmres {
         chunk = ((n + G - 1) / G + kThreads - 1) / kThreads * kThreads;
         if (chunk < kThreads) chunk = kThreads;
     }
-    ~Gmres()
+    int spmv(const double* x, double* y, const int* skip)
     {
-        if (hpart) (void)hipHostFree(hpart);
-    }
-    int init()
-    {
-        CEL_TRY(part.alloc(G));
-        CEL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hpart), sizeof(double) * G, hipHostMallocDefault));
-        return CEL_OK;
-    }
-    // sum of the workgroup partials in workgroup order
-    int reduce(double& out)
-    {
-        CEL_HIP(hipGetLastError());
-        CEL_HIP(hipMemcpyAsync(hpart, part.p, sizeof(double) * G, hipMemcpyDeviceToHost, st));
-        CEL_HIP(hipStreamSynchronize(st));
-        double s = 0.0;
-        for (int i = 0; i < G; ++i) s = s + hpart[i];
-        out = s;
-        return CEL_OK;
-    }
-    int spmv(const double* x, double* y)
-    {
-        ++launches;
-        if (format == CEL_FMT_JAD) return launch_jad_spmv(A, x, y, 0);
-        if (format == CEL_FMT_DIA) return launch_dia_spmv(A, x, y, 0);
-        return launch_csr_spmv(A, x, y, 0);
-    }
-    // r = b - A x, returns ||r||
-    int residual(const double* b, const double* x, double* r, double& norm)
-    {
-        CEL_TRY(spmv(x, r));
-        k_resid<<<G, kThreads, 0, st>>>(n, chunk, b, r, part.p);
-        double s;
-        CEL_TRY(reduce(s));
-        norm = std::sqrt(s);
-        return CEL_OK;
+        if (format == CEL_FMT_JAD) return launch_jad_spmv(A, x, y, 0, skip);
+        if (format == CEL_FMT_DIA) return launch_dia_spmv(A, x, y, 0, skip);
+        return launch_csr_spmv(A, x, y, 0, skip);
     }
 };
 
@@ -220,15 +350,18 @@ int gmres_solve(Mat& A, const double* b_in, double* x_out, const cel_gmres_param
     stats = cel_pcg_stats{};
     if (n == 0) return CEL_OK;
     Gmres S(A, prm.format);
-    CEL_TRY(S.init());
     hipStream_t st = S.st;
     const int G = S.G;
     const long long ck = S.chunk;
-    DevBuf<double> xb, bb, r, w, V, Z;
+    DevBuf<double> xb, bb, r, w, V, Z, partA, partB;
+    DevBuf<GmState> state;
     CEL_TRY(r.alloc(n));
     CEL_TRY(w.alloc(n));
-    CEL_TRY(V.alloc((size_t)n * (m + 1)));
+    CEL_TRY(V.alloc((size_t)n * m));
     CEL_TRY(Z.alloc((size_t)n * m));
+    CEL_TRY(partA.alloc(G));
+    CEL_TRY(partB.alloc(G));
+    CEL_TRY(state.alloc(1));
     const double* b = b_in;
     double* x = x_out;
     if (prm.memspace == CEL_MEM_HOST) {
@@ -240,124 +373,119 @@ int gmres_solve(Mat& A, const double* b_in, double* x_out, const cel_gmres_param
         b = bb.p;
         x = xb.p;
     }
+    GmState* gs = state.p;
+    const int* halt = &gs->halt;
+    const int* stop = &gs->stop;
     auto vv = [&](int k) { return V.p + (size_t)(k - 1) * n; };  // v_k, 1-based
     auto zz = [&](int k) { return Z.p + (size_t)(k - 1) * n; };
-    hipEvent_t e0, e1;
-    CEL_HIP(hipEventCreate(&e0));
-    CEL_HIP(hipEventCreate(&e1));
-    CEL_HIP(hipEventRecord(e0, st));
-    if (prm.x0_mode == CEL_X0_ONE || prm.x0_mode == CEL_X0_ZERO)
-        k_fill<<<G, kThreads, 0, st>>>(n, ck, x, prm.x0_mode == CEL_X0_ONE ? 1.0 : 0.0);
 
-    // 1-based small arrays as the reference's
-    const int M1 = m + 2;
-    std::vector<double> hh((size_t)M1 * M1, 0.0), cc(M1, 0.0), ss(M1, 0.0), gam(M1, 0.0), alp(M1, 0.0);
-    auto H = [&](int i, int j) -> double& { return hh[(size_t)i * M1 + j]; };
-    int conv = 0, iter = 0, cycles = 0;
-    double residuum = 0.0;
-    bool first = true;
-    while (iter <= prm.iter_max && conv == 0) {
-        iter += 1;
-        CEL_TRY(S.residual(b, x, r.p, residuum));
-        if (first) stats.norm0 = residuum;
-        first = false;
-        if (!std::isfinite(residuum)) {
-            stats.final_norm = residuum;
-            return fail(CEL_ERR_NONFINITE, "the residual is not finite");
-        }
-        gam[1] = residuum;
-        double local_residuum = residuum;
-        int restart_count = 0;
-        while (restart_count <= m && conv == 0) {
-            const double tmp = 1.0 / local_residuum;
-            k_scale_jac<<<G, kThreads, 0, st>>>(n, ck, r.p, tmp, A.jac.p, vv(1), zz(1));
-            gam[1] = residuum;
+    // one outer iteration: r = b - A x, then up to m+1 cycles
+    auto enqueue_outer = [&]() -> int {
+        CEL_TRY(S.spmv(x, r.p, stop));
+        k_resid<<<G, kThreads, 0, st>>>(n, ck, b, r.p, partA.p, gs);
+        k_outer_start<<<1, kThreads, 0, st>>>(partA.p, G, gs);
+        for (int c = 0; c <= m; ++c) {
+            k_scale_jac<<<G, kThreads, 0, st>>>(n, ck, r.p, A.jac.p, vv(1), zz(1), gs);
             for (int jj = 1; jj <= m; ++jj) {
-                CEL_TRY(S.spmv(zz(jj), w.p));
-                k_dot<<<G, kThreads, 0, st>>>(n, ck, vv(1), w.p, S.part.p);
-                CEL_TRY(S.reduce(H(1, jj)));
+                CEL_TRY(S.spmv(zz(jj), w.p, halt));
+                double* cur = partA.p;
+                double* nxt = partB.p;
+                k_dot<<<G, kThreads, 0, st>>>(n, ck, vv(1), w.p, cur, gs);
                 for (int ii = 1; ii <= jj; ++ii) {
                     const double* next = ii < jj ? vv(ii + 1) : nullptr;
-                    k_mgs<<<G, kThreads, 0, st>>>(n, ck, H(ii, jj), vv(ii), w.p, next, S.part.p);
-                    double s;
-                    CEL_TRY(S.reduce(s));
-                    if (ii < jj) H(ii + 1, jj) = s;
-                    else H(jj + 1, jj) = std::sqrt(s);
+                    k_mgs<<<G, kThreads, 0, st>>>(n, ck, ii, jj, cur, G, vv(ii), w.p, next, nxt, gs);
+                    std::swap(cur, nxt);
                 }
-                for (int ii = 1; ii <= jj - 1; ++ii) {  // as the reference writes it
-                    double a = H(ii, jj) * cc[ii];
-                    double c = H(ii + 1, jj) * ss[ii];
-                    H(ii, jj) = a + c;
-                    a = -H(ii, jj) * ss[ii];
-                    c = H(ii + 1, jj) * cc[ii];
-                    H(ii, jj) = a + c;
-                }
-                {
-                    const double a = H(jj, jj) * H(jj, jj);
-                    const double c = H(jj + 1, jj) * H(jj + 1, jj);
-                    const double betta = std::sqrt(a + c);
-                    ss[jj] = H(jj + 1, jj) / betta;
-                    cc[jj] = H(jj, jj) / betta;
-                    H(jj, jj) = betta;
-                    gam[jj + 1] = -ss[jj] * gam[jj];
-                    gam[jj] = cc[jj] * gam[jj];
-                }
-                if (std::fabs(gam[jj + 1]) <= prm.eps) {
-                    Coefs c{};
-                    for (int ii = jj; ii >= 1; --ii) {
-                        alp[ii] = 0.0;
-                        for (int kk = ii + 1; kk <= jj; ++kk) {
-                            const double t = H(ii, kk) * alp[kk];
-                            alp[ii] = alp[ii] + t;
-                        }
-                        const double inv = 1.0 / H(ii, ii);
-                        alp[ii] = inv * (gam[ii] - alp[ii]);
-                    }
-                    for (int ii = 1; ii <= jj; ++ii) c.a[ii - 1] = alp[ii];
-                    k_xupdate<<<G, kThreads, 0, st>>>(n, ck, jj, c, Z.p, n, x);
-                    conv = 2;
-                    break;
-                }
-                // z_m+1 is never used
-                k_div_jac<<<G, kThreads, 0, st>>>(n, ck, w.p, H(jj + 1, jj), A.jac.p, vv(jj + 1),
-                                                  jj < m ? zz(jj + 1) : nullptr);
+                k_givens<<<1, kThreads, 0, st>>>(jj, m, cur, G, gs);
+                k_next<<<G, kThreads, 0, st>>>(n, ck, jj, m, w.p, A.jac.p, V.p, Z.p, x, gs);
             }
-            if (conv == 0) {
-                Coefs c{};
-                for (int ii = m; ii >= 1; --ii) {
-                    alp[ii] = 0.0;
-                    for (int kk = ii + 1; kk <= m; ++kk) {
-                        const double t = H(ii, kk) * alp[kk];
-                        alp[ii] = alp[ii] + t;
-                    }
-                    const double inv = 1.0 / H(ii, ii);
-                    alp[ii] = inv * (gam[ii] - alp[ii]);
-                }
-                for (int ii = 1; ii <= m; ++ii) c.a[ii - 1] = alp[ii];
-                k_xupdate<<<G, kThreads, 0, st>>>(n, ck, m, c, Z.p, n, x);
-            }
-            CEL_HIP(hipGetLastError());
-            CEL_TRY(S.residual(b, x, r.p, residuum));
-            local_residuum = residuum;
-            cycles += 1;
-            restart_count += 1;
+            CEL_TRY(S.spmv(x, r.p, stop));
+            k_resid<<<G, kThreads, 0, st>>>(n, ck, b, r.p, partA.p, gs);
+            k_cycle_end<<<1, kThreads, 0, st>>>(partA.p, G, gs);
+        }
+        CEL_HIP(hipGetLastError());
+        return CEL_OK;
+    };
+
+    hipEvent_t e0, e1, cev[2];
+    CEL_HIP(hipEventCreate(&e0));
+    CEL_HIP(hipEventCreate(&e1));
+    CEL_HIP(hipEventCreateWithFlags(&cev[0], hipEventDisableTiming));
+    CEL_HIP(hipEventCreateWithFlags(&cev[1], hipEventDisableTiming));
+    int* hstop = nullptr;
+    CEL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hstop), 2 * sizeof(int), hipHostMallocDefault));
+    hstop[0] = hstop[1] = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int rc = CEL_OK;
+
+    CEL_HIP(hipEventRecord(e0, st));
+    k_state_init<<<1, 1, 0, st>>>(gs, prm.eps);
+    if (prm.x0_mode == CEL_X0_ONE || prm.x0_mode == CEL_X0_ZERO)
+        k_fill<<<G, kThreads, 0, st>>>(n, ck, x, prm.x0_mode == CEL_X0_ONE ? 1.0 : 0.0);
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        rc = fail(CEL_ERR_HIP, "graph capture could not start");
+    } else {
+        rc = enqueue_outer();
+        const hipError_t ec = hipStreamEndCapture(st, &graph);
+        if (rc >= 0 && ec != hipSuccess)
+            rc = fail(CEL_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+        if (rc >= 0) {
+            const hipError_t ei = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+            if (ei != hipSuccess)
+                rc = fail(CEL_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
         }
     }
-    CEL_HIP(hipEventRecord(e1, st));
-    if (prm.memspace == CEL_MEM_HOST)
-        CEL_HIP(hipMemcpyAsync(x_out, x, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    CEL_HIP(hipStreamSynchronize(st));
+    // outer iterations: the reference runs them while iter <= iter_max
+    // (iter counted from 1), so at most iter_max + 1
+    const long long budget = (long long)prm.iter_max + 1;
+    for (long long k = 0; rc >= 0 && k < budget; ++k) {
+        if (hipGraphLaunch(gexec, st) != hipSuccess ||
+            hipMemcpyAsync(&hstop[k & 1], stop, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipEventRecord(cev[k & 1], st) != hipSuccess) {
+            rc = fail(CEL_ERR_HIP, "GMRES: graph launch failed");
+            break;
+        }
+        if (k > 0) {
+            if (hipEventSynchronize(cev[(k - 1) & 1]) != hipSuccess) {
+                rc = fail(CEL_ERR_HIP, "GMRES: event wait failed");
+                break;
+            }
+            if (hstop[(k - 1) & 1]) break;
+        }
+    }
+    (void)hipEventRecord(e1, st);
+    if (rc >= 0 && prm.memspace == CEL_MEM_HOST)
+        (void)hipMemcpyAsync(x_out, x, sizeof(double) * n, hipMemcpyDeviceToHost, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    GmState out{};
+    if (rc >= 0 && es == hipSuccess && hipMemcpy(&out, gs, sizeof(GmState), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(CEL_ERR_HIP, "GMRES: state copy failed");
     float ms = 0.f;
-    CEL_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipEventDestroy(cev[0]);
+    (void)hipEventDestroy(cev[1]);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    stats.iters = iter;
-    stats.outer_iters = cycles;
-    stats.final_norm = residuum;
+    (void)hipHostFree(hstop);
+    if (rc < 0) return rc;
+    if (es != hipSuccess) return fail(CEL_ERR_HIP, std::string("GMRES: ") + hipGetErrorString(es));
+
+    stats.iters = out.iter;
+    stats.outer_iters = out.cycles;
+    stats.norm0 = out.norm0;
+    stats.final_norm = out.residuum;
     stats.eps_abs = prm.eps;
     stats.solve_ms = ms;
-    stats.spmv_launches = S.launches;
-    stats.status = conv == 2 ? CEL_OK : CEL_NOT_CONVERGED;
+    // products run: one per outer residual, per cycle residual, per column
+    stats.spmv_launches = out.iter + out.cycles + out.ncols;
+    if (out.status != 0) {
+        stats.status = out.status;
+        return fail(out.status, "the residual is not finite");
+    }
+    stats.status = out.conv_col ? CEL_OK : CEL_NOT_CONVERGED;
     return stats.status;
 }
 

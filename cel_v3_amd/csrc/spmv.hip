@@ -13,8 +13,10 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmv(int n, const int* __restric
                                                      const int* __restrict__ cols,
                                                      const VT* __restrict__ vals,
                                                      const VT* __restrict__ x, VT* __restrict__ y,
-                                                     int accumulate, int ntiles, int tpb)
+                                                     int accumulate, int ntiles, int tpb,
+                                                     const int* __restrict__ skip)
 {
+    if (skip && *skip) return;  // uniform across the grid
     __shared__ CsrStage<VT> sm;
     const int lb = logical_block();
     const int t0 = lb * tpb, t1 = min(t0 + tpb, ntiles);
@@ -43,8 +45,10 @@ __global__ __launch_bounds__(kBlock) void k_jad_spmv(int n, int mmax, const int*
                                                      const VT* __restrict__ jvals,
                                                      const int* __restrict__ perm,
                                                      const VT* __restrict__ x, VT* __restrict__ y,
-                                                     int accumulate, int ntiles, int tpb)
+                                                     int accumulate, int ntiles, int tpb,
+                                                     const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int lb = logical_block();
     const int t0 = lb * tpb, t1 = min(t0 + tpb, ntiles);
     for (int t = t0; t < t1; ++t) {
@@ -67,14 +71,14 @@ static int spmv_no_columns(const Mat& A, T* y, int accumulate)
     return CEL_OK;
 }
 
-int launch_csr_spmv(const Mat& A, const double* x, double* y, int accumulate)
+int launch_csr_spmv(const Mat& A, const double* x, double* y, int accumulate, const int* skip)
 {
     if (A.n == 0) return CEL_OK;
     if (A.ncols == 0) return spmv_no_columns(A, y, accumulate);
     int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int G = resident_grid((const void*)k_csr_spmv<double>, ntiles);
     hipLaunchKernelGGL(k_csr_spmv<double>, dim3(G), dim3(kBlock), 0, rt().stream, A.n, A.xadj.p,
-                       A.cols.p, A.vals.p, x, y, accumulate, ntiles, tiles_per_block(ntiles, G));
+                       A.cols.p, A.vals.p, x, y, accumulate, ntiles, tiles_per_block(ntiles, G), skip);
     CEL_HIP(hipGetLastError());
     return CEL_OK;
 }
@@ -87,19 +91,20 @@ int launch_csr_spmv_f32(const Mat& A, const float* x, float* y, int accumulate)
     int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int G = resident_grid((const void*)k_csr_spmv<float>, ntiles);
     hipLaunchKernelGGL(k_csr_spmv<float>, dim3(G), dim3(kBlock), 0, rt().stream, A.n, A.xadj.p,
-                       A.cols.p, A.vals32.p, x, y, accumulate, ntiles, tiles_per_block(ntiles, G));
+                       A.cols.p, A.vals32.p, x, y, accumulate, ntiles, tiles_per_block(ntiles, G),
+                       nullptr);
     CEL_HIP(hipGetLastError());
     return CEL_OK;
 }
 
-int launch_jad_spmv(const Mat& A, const double* x, double* y, int accumulate)
+int launch_jad_spmv(const Mat& A, const double* x, double* y, int accumulate, const int* skip)
 {
     if (A.n == 0) return CEL_OK;
     int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int G = resident_grid((const void*)k_jad_spmv<double>, ntiles);
     hipLaunchKernelGGL(k_jad_spmv<double>, dim3(G), dim3(kBlock), 0, rt().stream, A.n, A.jad_mmax,
                        A.jad_begin.p, A.jad_cols.p, A.jad_vals.p, A.jad_perm.p, x, y, accumulate,
-                       ntiles, tiles_per_block(ntiles, G));
+                       ntiles, tiles_per_block(ntiles, G), skip);
     CEL_HIP(hipGetLastError());
     return CEL_OK;
 }
@@ -112,7 +117,7 @@ int launch_jad_spmv_f32(const Mat& A, const float* x, float* y, int accumulate)
     int G = resident_grid((const void*)k_jad_spmv<float>, ntiles);
     hipLaunchKernelGGL(k_jad_spmv<float>, dim3(G), dim3(kBlock), 0, rt().stream, A.n, A.jad_mmax,
                        A.jad_begin.p, A.jad_cols.p, A.jad_vals32.p, A.jad_perm.p, x, y, accumulate,
-                       ntiles, tiles_per_block(ntiles, G));
+                       ntiles, tiles_per_block(ntiles, G), nullptr);
     CEL_HIP(hipGetLastError());
     return CEL_OK;
 }
