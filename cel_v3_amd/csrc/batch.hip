@@ -714,9 +714,10 @@ static int run_batch(Batch& B, const Tuning& T, const cel_pcg_params& prm, int i
     BCtrl* C1 = B.ctrl.p + B.lanes;
     BCtrl* C2 = B.ctrl.p + 2 * B.lanes;
 
+    StreamScope R(s);
     hipEvent_t e0, e1;
-    CEL_HIP(hipEventCreate(&e0));
-    CEL_HIP(hipEventCreate(&e1));
+    CEL_TRY(R.event(e0));
+    CEL_TRY(R.event(e1));
     CEL_HIP(hipEventRecord(e0, s));
     hipLaunchKernelGGL(k_bresidual, dim3(G), dim3(kBlock), 0, s, B.d_sys.p, B.d_tbl.p, B.x.p, B.b.p,
                        B.r.p, B.p.p, B.rpart.p, B.kpmax, G, C0);
@@ -736,24 +737,23 @@ static int run_batch(Batch& B, const Tuning& T, const cel_pcg_params& prm, int i
     // chunks of T.chunk iterations replayed from one captured graph; the
     // host checks the lanes one chunk behind and re-deals the blocks of
     // stopped systems
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
     CEL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < T.chunk; ++i) {
         int rc = enqueue_iter();
         if (rc < 0) {
-            (void)hipStreamEndCapture(s, &graph);
+            (void)hipStreamEndCapture(s, &R.graph);
             return rc;
         }
     }
-    CEL_HIP(hipStreamEndCapture(s, &graph));
-    CEL_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    CEL_HIP(hipStreamEndCapture(s, &R.graph));
+    CEL_HIP(hipGraphInstantiate(&R.exec, R.graph, nullptr, nullptr, 0));
+    hipGraphExec_t gexec = R.exec;
 
     BCtrl* hc = nullptr;
-    CEL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hc), 2 * sizeof(BCtrl) * B.lanes, hipHostMallocDefault));
+    CEL_TRY(R.host_alloc(hc, 2 * (size_t)B.lanes));
     hipEvent_t cev[2];
-    CEL_HIP(hipEventCreateWithFlags(&cev[0], hipEventDisableTiming));
-    CEL_HIP(hipEventCreateWithFlags(&cev[1], hipEventDisableTiming));
+    CEL_TRY(R.event(cev[0], hipEventDisableTiming));
+    CEL_TRY(R.event(cev[1], hipEventDisableTiming));
     const long long budget = (long long)iter_cap + 1;
     long long enq = 0;
     int k = 0;
@@ -793,13 +793,6 @@ static int run_batch(Batch& B, const Tuning& T, const cel_pcg_params& prm, int i
     CEL_HIP(hipEventElapsedTime(&out.ms, e0, e1));
     out.lanes.resize(B.lanes);
     CEL_HIP(hipMemcpy(out.lanes.data(), C0, sizeof(BCtrl) * B.lanes, hipMemcpyDeviceToHost));
-    (void)hipGraphExecDestroy(gexec);
-    (void)hipGraphDestroy(graph);
-    (void)hipEventDestroy(cev[0]);
-    (void)hipEventDestroy(cev[1]);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipHostFree(hc);
     return rc;
 }
 

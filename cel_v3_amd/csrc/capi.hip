@@ -555,17 +555,16 @@ extern "C" int cel_gpu_spmv_time(cel_gpu_mat* A, int format, int precision, cons
         }
     }
     CEL_TRY(once());  // warm-up
+    StreamScope R(s);
     hipEvent_t a, b;
-    CEL_HIP(hipEventCreate(&a));
-    CEL_HIP(hipEventCreate(&b));
+    CEL_TRY(R.event(a));
+    CEL_TRY(R.event(b));
     CEL_HIP(hipEventRecord(a, s));
     for (int i = 0; i < reps; ++i) CEL_TRY(once());
     CEL_HIP(hipEventRecord(b, s));
     CEL_HIP(hipEventSynchronize(b));
     float ms = 0;
     CEL_HIP(hipEventElapsedTime(&ms, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     if (mean_ms) *mean_ms = ms / reps;
     if (bytes) *bytes = spmv_bytes(M, format, precision);
     return CEL_OK;
@@ -793,9 +792,10 @@ extern "C" int cel_gpu_stream_copy_time(size_t bytes, int reps, double* mean_ms)
     CEL_HIP(hipMemsetAsync(a.p, 0, n * 32, s));
     const int G = rt().num_cus * 32;
     hipLaunchKernelGGL(k_stream_copy, dim3(G), dim3(256), 0, s, n, a.p, b.p);
+    StreamScope R(s);
     hipEvent_t e0, e1;
-    CEL_HIP(hipEventCreate(&e0));
-    CEL_HIP(hipEventCreate(&e1));
+    CEL_TRY(R.event(e0));
+    CEL_TRY(R.event(e1));
     CEL_HIP(hipEventRecord(e0, s));
     for (int i = 0; i < reps; ++i)
         hipLaunchKernelGGL(k_stream_copy, dim3(G), dim3(256), 0, s, n, a.p, b.p);
@@ -803,8 +803,6 @@ extern "C" int cel_gpu_stream_copy_time(size_t bytes, int reps, double* mean_ms)
     CEL_HIP(hipEventSynchronize(e1));
     float ms = 0;
     CEL_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (mean_ms) *mean_ms = ms / reps;
     return CEL_OK;
 }

@@ -73,6 +73,44 @@ struct DevBuf {
     }
 };
 
+// The events, pinned host buffers and graph one host routine creates on a
+// stream, released on every return path once the stream has drained (so no
+// copy still targets a pinned buffer).
+struct StreamScope {
+    hipStream_t s;
+    std::vector<hipEvent_t> ev;
+    std::vector<void*> pinned;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    explicit StreamScope(hipStream_t st) : s(st) {}
+    StreamScope(const StreamScope&) = delete;
+    StreamScope& operator=(const StreamScope&) = delete;
+    ~StreamScope()
+    {
+        (void)hipStreamSynchronize(s);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        for (void* q : pinned) (void)hipHostFree(q);
+    }
+    int event(hipEvent_t& e, unsigned flags = hipEventDefault)
+    {
+        if (hipEventCreateWithFlags(&e, flags) != hipSuccess) return fail(CEL_ERR_HIP, "hipEventCreate failed");
+        ev.push_back(e);
+        return CEL_OK;
+    }
+    template <typename T>
+    int host_alloc(T*& p, size_t count)
+    {
+        void* q = nullptr;
+        if (hipHostMalloc(&q, sizeof(T) * (count > 0 ? count : 1), hipHostMallocDefault) != hipSuccess)
+            return fail(CEL_ERR_OOM, "hipHostMalloc failed");
+        pinned.push_back(q);
+        p = static_cast<T*>(q);
+        return CEL_OK;
+    }
+};
+
 // ---------------------------------------------------------------- matrix
 struct Workspace;  // PCG work vectors, owned by the matrix handle (pcg.hip)
 // One halo block: entries of local rows whose columns belong to another

@@ -407,71 +407,48 @@ int gmres_solve(Mat& A, const double* b_in, double* x_out, const cel_gmres_param
         return CEL_OK;
     };
 
+    StreamScope R(st);
     hipEvent_t e0, e1, cev[2];
-    CEL_HIP(hipEventCreate(&e0));
-    CEL_HIP(hipEventCreate(&e1));
-    CEL_HIP(hipEventCreateWithFlags(&cev[0], hipEventDisableTiming));
-    CEL_HIP(hipEventCreateWithFlags(&cev[1], hipEventDisableTiming));
+    CEL_TRY(R.event(e0));
+    CEL_TRY(R.event(e1));
+    CEL_TRY(R.event(cev[0], hipEventDisableTiming));
+    CEL_TRY(R.event(cev[1], hipEventDisableTiming));
     int* hstop = nullptr;
-    CEL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hstop), 2 * sizeof(int), hipHostMallocDefault));
+    CEL_TRY(R.host_alloc(hstop, 2));
     hstop[0] = hstop[1] = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int rc = CEL_OK;
 
     CEL_HIP(hipEventRecord(e0, st));
     k_state_init<<<1, 1, 0, st>>>(gs, prm.eps);
     if (prm.x0_mode == CEL_X0_ONE || prm.x0_mode == CEL_X0_ZERO)
         k_fill<<<G, kThreads, 0, st>>>(n, ck, x, prm.x0_mode == CEL_X0_ONE ? 1.0 : 0.0);
-    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        rc = fail(CEL_ERR_HIP, "graph capture could not start");
-    } else {
-        rc = enqueue_outer();
-        const hipError_t ec = hipStreamEndCapture(st, &graph);
-        if (rc >= 0 && ec != hipSuccess)
-            rc = fail(CEL_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
-        if (rc >= 0) {
-            const hipError_t ei = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-            if (ei != hipSuccess)
-                rc = fail(CEL_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
-        }
+    CEL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    {
+        const int rc = enqueue_outer();
+        const hipError_t ec = hipStreamEndCapture(st, &R.graph);
+        if (rc < 0) return rc;
+        CEL_HIP(ec);
     }
+    CEL_HIP(hipGraphInstantiate(&R.exec, R.graph, nullptr, nullptr, 0));
     // outer iterations: the reference runs them while iter <= iter_max
     // (iter counted from 1), so at most iter_max + 1
     const long long budget = (long long)prm.iter_max + 1;
-    for (long long k = 0; rc >= 0 && k < budget; ++k) {
-        if (hipGraphLaunch(gexec, st) != hipSuccess ||
-            hipMemcpyAsync(&hstop[k & 1], stop, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipEventRecord(cev[k & 1], st) != hipSuccess) {
-            rc = fail(CEL_ERR_HIP, "GMRES: graph launch failed");
-            break;
-        }
+    for (long long k = 0; k < budget; ++k) {
+        CEL_HIP(hipGraphLaunch(R.exec, st));
+        CEL_HIP(hipMemcpyAsync(&hstop[k & 1], stop, sizeof(int), hipMemcpyDeviceToHost, st));
+        CEL_HIP(hipEventRecord(cev[k & 1], st));
         if (k > 0) {
-            if (hipEventSynchronize(cev[(k - 1) & 1]) != hipSuccess) {
-                rc = fail(CEL_ERR_HIP, "GMRES: event wait failed");
-                break;
-            }
+            CEL_HIP(hipEventSynchronize(cev[(k - 1) & 1]));
             if (hstop[(k - 1) & 1]) break;
         }
     }
-    (void)hipEventRecord(e1, st);
-    if (rc >= 0 && prm.memspace == CEL_MEM_HOST)
-        (void)hipMemcpyAsync(x_out, x, sizeof(double) * n, hipMemcpyDeviceToHost, st);
-    const hipError_t es = hipStreamSynchronize(st);
+    CEL_HIP(hipEventRecord(e1, st));
+    if (prm.memspace == CEL_MEM_HOST)
+        CEL_HIP(hipMemcpyAsync(x_out, x, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    CEL_HIP(hipStreamSynchronize(st));
     GmState out{};
-    if (rc >= 0 && es == hipSuccess && hipMemcpy(&out, gs, sizeof(GmState), hipMemcpyDeviceToHost) != hipSuccess)
-        rc = fail(CEL_ERR_HIP, "GMRES: state copy failed");
+    CEL_HIP(hipMemcpy(&out, gs, sizeof(GmState), hipMemcpyDeviceToHost));
     float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
-    (void)hipEventDestroy(cev[0]);
-    (void)hipEventDestroy(cev[1]);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipHostFree(hstop);
-    if (rc < 0) return rc;
-    if (es != hipSuccess) return fail(CEL_ERR_HIP, std::string("GMRES: ") + hipGetErrorString(es));
+    CEL_HIP(hipEventElapsedTime(&ms, e0, e1));
 
     stats.iters = out.iter;
     stats.outer_iters = out.cycles;

@@ -911,10 +911,11 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         V.ghost = W.ghost.p;
     }
 
+    StreamScope R(s);
     hipEvent_t e0, e1, e2;
-    CEL_HIP(hipEventCreate(&e0));
-    CEL_HIP(hipEventCreate(&e1));
-    CEL_HIP(hipEventCreate(&e2));
+    CEL_TRY(R.event(e0));
+    CEL_TRY(R.event(e1));
+    CEL_TRY(R.event(e2));
     CEL_HIP(hipEventRecord(e0, s));
 
     // x0 (cel_cgalg_module.f90:209 uses 1.0)
@@ -955,7 +956,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     auto prof_event = [&]() -> hipEvent_t {
         if (pev_used == (int)pev.size()) {
             hipEvent_t ev;
-            if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+            if (R.event(ev) < 0) return nullptr;
             pev.push_back(ev);
         }
         return pev[pev_used++];
@@ -1025,14 +1026,12 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     // iter_max iterations plus one K1 that folds in the final partials.
     const int chunk = prm.check_every > 0 ? prm.check_every : 16;
     int* hrun = nullptr;
-    CEL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hrun), 2 * sizeof(int), hipHostMallocDefault));
+    CEL_TRY(R.host_alloc(hrun, 2));
     hrun[0] = hrun[1] = 1;
     hipEvent_t cev[2];
-    CEL_HIP(hipEventCreateWithFlags(&cev[0], hipEventDisableTiming));
-    CEL_HIP(hipEventCreateWithFlags(&cev[1], hipEventDisableTiming));
+    CEL_TRY(R.event(cev[0], hipEventDisableTiming));
+    CEL_TRY(R.event(cev[1], hipEventDisableTiming));
 
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
     const bool use_graph = prm.use_graph && !prm.profile;
     int status = CEL_OK;
     const long long budget = (long long)prm.iter_max + 1;
@@ -1041,19 +1040,19 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     while (enqueued < budget) {
         const int c = (int)std::min<long long>(chunk, budget - enqueued);
         if (use_graph && c == chunk) {
-            if (!gexec) {
+            if (!R.exec) {
                 CEL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 for (int i = 0; i < c; ++i) {
                     int rc = enqueue_iter(false);
                     if (rc < 0) {
-                        (void)hipStreamEndCapture(s, &graph);
+                        (void)hipStreamEndCapture(s, &R.graph);
                         return rc;
                     }
                 }
-                CEL_HIP(hipStreamEndCapture(s, &graph));
-                CEL_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+                CEL_HIP(hipStreamEndCapture(s, &R.graph));
+                CEL_HIP(hipGraphInstantiate(&R.exec, R.graph, nullptr, nullptr, 0));
             }
-            CEL_HIP(hipGraphLaunch(gexec, s));
+            CEL_HIP(hipGraphLaunch(R.exec, s));
         } else {
             for (int i = 0; i < c; ++i) {
                 status = enqueue_iter(prm.profile != 0);
@@ -1076,11 +1075,6 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     }
     CEL_HIP(hipEventRecord(e2, s));
     CEL_HIP(hipStreamSynchronize(s));
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
-    (void)hipEventDestroy(cev[0]);
-    (void)hipEventDestroy(cev[1]);
-    (void)hipHostFree(hrun);
     if (status < 0) return status;
 
     Ctrl out;
@@ -1097,10 +1091,6 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
             ++spmv_n;
         }
     }
-    for (auto ev : pev) (void)hipEventDestroy(ev);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
 
     if (prm.memspace == CEL_MEM_HOST)
         CEL_HIP(hipMemcpy(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost));
