@@ -3,8 +3,9 @@
 //
 // CSR tile (CSR-stream): one 256-row tile per block iteration.  The tile's
 // entries [xadj[r0], xadj[r0+256]) are contiguous, so the block streams
-// values and column indices with coalesced loads into registers (8 entries
-// per thread, all loads issued before the first wait), parks them in LDS,
+// values and column indices with coalesced 16-byte loads into registers (8
+// entries per thread, all loads issued before the first wait; measured 6-7%
+// faster than 4- and 8-byte loads on the 256^3 7-point block), parks them in LDS,
 // and every thread then sums *its own row* from LDS in column order.  That
 // keeps the reference's per-row summation order
 // (src/fop/cel_fop_mv_module.f90:215-229) while the HBM stream stays fully
@@ -29,6 +30,9 @@ namespace cel {
 constexpr int kTileRows = kBlock;
 constexpr int kStageCap = 2048;            // staged entries per tile (8 per row on average)
 constexpr int kPer = kStageCap / kBlock;   // staged entries per thread
+// entries of a tile that can be staged: the 16-byte loads of a tile start
+// up to 3 entries before it (at a 16-byte boundary)
+constexpr int kStageMax = kStageCap - 4;
 constexpr int kGather = 8;                 // x values gathered per round trip
 
 template <typename VT>
@@ -159,10 +163,12 @@ __device__ __forceinline__ void dia_rows(int n, int nd, const int* doff, const V
 //   for t: s = rows(t, next = t+1); sync; stage; sync
 template <typename VT>
 struct CsrTileStream {
+    static constexpr int EV = 16 / (int)sizeof(VT);  // values per 16-byte load
     VT v[kPer];
     int c[kPer];
     int e0, cnt;  // first entry and entries of the fetched tile (uniform)
     int lo, hi;   // the calling thread's row bounds (lo = hi = e0 past n)
+    int sv, sc;   // entries the 16-byte value / column loads start before e0
 
     // Loads of tile t.  Unconditional by design: with want = false (no next
     // tile; t is then the current tile) or a tile larger than the stage,
@@ -181,28 +187,55 @@ struct CsrTileStream {
         // threads past the last row get lo = hi (an empty row)
         lo = xadj[r0 + min(tid, nrows)];
         hi = xadj[r0 + min(tid + 1, nrows)];
-        const int m = (want && cnt <= kStageCap) ? cnt : 0;
-        const auto rv = tile_rsrc(vals + e0, m * (int)sizeof(VT));
-        const auto rc = tile_rsrc(cols + e0, m * (int)sizeof(int));
+        const int m = (want && cnt <= kStageMax) ? cnt : 0;
+        // 16-byte loads from the 16-byte boundary at or before e0: thread
+        // tid holds the EV (values) or 4 (columns) consecutive entries at
+        // EV*tid + EV*256*i (4*tid + 1024*i) counted from that boundary
+        sv = e0 % EV;
+        sc = e0 % 4;
+        const auto rv = tile_rsrc(vals + (e0 - sv), m > 0 ? (m + sv) * (int)sizeof(VT) : 0);
+        const auto rc = tile_rsrc(cols + (e0 - sc), m > 0 ? (m + sc) * (int)sizeof(int) : 0);
 #pragma unroll
-        for (int i = 0; i < kPer; ++i) {
-            v[i] = buf_load_nt<VT>(rv, tid * (int)sizeof(VT), i * kBlock * (int)sizeof(VT));
-            c[i] = buf_load_nt<int>(rc, tid * (int)sizeof(int), i * kBlock * (int)sizeof(int));
+        for (int i = 0; i < kPer / EV; ++i) {
+            const auto w = __builtin_amdgcn_raw_buffer_load_b128(rv, 16 * tid, 16 * kBlock * i, 2);
+#pragma unroll
+            for (int j = 0; j < EV; ++j) {
+                if constexpr (sizeof(VT) == 8) {
+                    const unsigned long long u =
+                        (unsigned long long)(unsigned)w[2 * j] | ((unsigned long long)(unsigned)w[2 * j + 1] << 32);
+                    v[i * EV + j] = __builtin_bit_cast(VT, u);
+                } else {
+                    const unsigned u = (unsigned)w[j];  // a plain value, not the vector element
+                    v[i * EV + j] = __builtin_bit_cast(VT, u);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kPer / 4; ++i) {
+            const auto w = __builtin_amdgcn_raw_buffer_load_b128(rc, 16 * tid, 16 * kBlock * i, 2);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) c[i * 4 + j] = (int)w[j];
         }
     }
 
     // Parks the fetched entries in LDS (tiles that fit the stage).
     __device__ __forceinline__ void stage(CsrStage<VT>& sm)
     {
-        if (cnt <= kStageCap) {  // uniform across the block
+        if (cnt <= kStageMax) {  // uniform across the block
 #pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const int k = threadIdx.x + i * kBlock;
-                if (k < cnt) {
-                    sm.val[k] = v[i];
-                    sm.col[k] = c[i];
+            for (int i = 0; i < kPer / EV; ++i)
+#pragma unroll
+                for (int j = 0; j < EV; ++j) {
+                    const int k = EV * (int)threadIdx.x + EV * kBlock * i + j - sv;
+                    if (k >= 0 && k < cnt) sm.val[k] = v[i * EV + j];
                 }
-            }
+#pragma unroll
+            for (int i = 0; i < kPer / 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int k = 4 * (int)threadIdx.x + 4 * kBlock * i + j - sc;
+                    if (k >= 0 && k < cnt) sm.col[k] = c[i * 4 + j];
+                }
         }
     }
 
@@ -230,7 +263,7 @@ struct CsrTileStream {
                                        VT* xrow, Pre&& pre)
     {
         const int b0 = e0, l = lo - e0, len = hi - lo;
-        const bool staged = cnt <= kStageCap;  // uniform across the block
+        const bool staged = cnt <= kStageMax;  // uniform across the block
         // branch-free gathers: entries past the row read a clamped LDS slot
         // and x[0]; their products are not added
         VT xv[kGather];
