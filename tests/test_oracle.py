@@ -72,7 +72,8 @@ def test_poisson7_matches_brute_force(shape):
         assert np.all(np.diff(seg) > 0)
 
 
-def test_poisson7_nnz_baseline_sizes():
+def test_poisson7_nnz_at_32_256_384_cubed():
+    # 7 N^3 - 6 N^2 entries
     assert O.poisson7_nnz(32, 32, 32) == 223232
     assert O.poisson7_nnz(256, 256, 256) == 117047296
     assert O.poisson7_nnz(384, 384, 384) == 395476992
