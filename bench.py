@@ -200,6 +200,33 @@ def batch_bench(cel, steps):
             "speedup": seq_ms / float(np.median(times)), "tune_s": tune_s}
 
 
+def gmres_bench(cel, N, steps, iter_cap):
+    """Jacobi-GMRES with the element solver's settings (m = 3, eps 1e-8 on
+    the true residual norm, x0 = 1) on the N^3 7-point block, b ~ U[0,1)
+    (numpy default_rng(55)), inputs resident in HBM, in the DIA and the CSR
+    form.  iter_cap > 0 caps the outer iterations (for profiler runs)."""
+    M = cel.Matrix.poisson7(N, N, N)
+    M.set_jacobi()
+    M.to_dia()
+    b = np.random.default_rng(55).random(M.n)
+    d_b = cel.DeviceArray.from_host(b)
+    cap = iter_cap if iter_cap > 0 else 5000
+    out = {"workload": f"poisson7_{N}^3_jacobi_gmres3_eps1e-8", "n": M.n, "nnz": M.nnz}
+    for name, f in (("dia", cel.FMT_DIA), ("csr", cel.FMT_CSR)):
+        M.gmres(d_b, iter_max=min(cap, 2), fmt=f)  # warm-up (graph capture, first touch)
+        runs = [M.gmres(d_b, iter_max=cap, fmt=f) for _ in range(max(1, steps))]
+        r = runs[-1]
+        res = d_b.to_host() - M.spmv(r["x"].to_host())
+        ms = [x["solve_ms"] for x in runs]
+        out[name] = {"solve_ms": ms, "iters": r["iters"], "cycles": r["cycles"],
+                     "us_per_cycle": float(np.median(ms)) / max(r["cycles"], 1) * 1e3,
+                     "final_norm": r["final_norm"], "true_residual_norm": float(np.linalg.norm(res)),
+                     "status": r["status"]}
+        log(f"gmres {name}: {np.median(ms):.1f} ms, {r['iters']} outer iterations, {r['cycles']} cycles, "
+            f"{out[name]['us_per_cycle']:.1f} us per cycle, |r| {r['final_norm']:.3e}")
+    return out
+
+
 def sort_bench(cel, N, steps, warmup, skip_cpu):
     """COO ordering in HBM: the N^3 7-point matrix in shuffled entry order
     (numpy default_rng(7) permutation; the order a Matrix Market file need
@@ -280,14 +307,17 @@ def main():
     ap.add_argument("--format", default="dia", choices=["dia", "csr", "jad"])
     ap.add_argument("--skip-cpu", action="store_true", help="do not time the CPU baseline")
     ap.add_argument("--only", default="all",
-                    choices=["all", "fp64", "mixed", "spmv", "profile", "batch", "sort"],
+                    choices=["all", "fp64", "mixed", "spmv", "profile", "batch", "sort", "gmres"],
                     help="run one part only; 'profile' = one streaming copy of known size "
                          "(counter calibration) and FP64, FP64-with-halo and mixed solves "
                          "capped at --iters iterations, for hardware-counter runs; 'batch' = "
                          "the batched workload (64 variable-coefficient systems + 128^3 with 16 "
                          "right-hand sides) against one-by-one solves; 'sort' = COO ordering of "
-                         "the shuffled --size^3 matrix in HBM")
-    ap.add_argument("--iters", type=int, default=20, help="iteration cap of --only profile")
+                         "the shuffled --size^3 matrix in HBM; 'gmres' = Jacobi-GMRES(3) with "
+                         "the element solver's settings on the 128^3 block (--iters > 0 caps "
+                         "its outer iterations)")
+    ap.add_argument("--iters", type=int, default=None,
+                    help="iteration cap of --only profile (default 20) and --only gmres (default none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -300,6 +330,10 @@ def main():
     if args.only == "batch":
         cel.init(0)
         print(json.dumps(batch_bench(cel, args.steps)))
+        return
+    if args.only == "gmres":
+        cel.init(0)
+        print(json.dumps(gmres_bench(cel, 128, args.steps, args.iters or 0)))
         return
     if args.only == "sort":
         cel.init(0)
@@ -365,7 +399,7 @@ def main():
                      ("fp64_halo_csr", cel.PREC_FP64, use_halo, cel.FMT_CSR)]
         for name, prec, halo, f in runs:
             r = M.pcg(d_b, x0=None, tol=args.tol, tol_mode=cel.TOL_REL_B, x0_mode=cel.X0_ZERO,
-                      iter_max=args.iters, precision=prec, fmt=f, use_halo=halo, profile=1)
+                      iter_max=args.iters or 20, precision=prec, fmt=f, use_halo=halo, profile=1)
             prof[name] = {"iters": r["iters"], "spmv_bytes": r["spmv_bytes"],
                           "mean_spmv_ms": r["spmv_ms"] / max(r["spmv_launches"], 1)}
             log(f"profile {name}: {r['iters']} iterations, K1 "

@@ -95,13 +95,14 @@ k_fill(long long n, long long chunk, double* __restrict__ x, double v)
     CHUNK_LOOP(i) x[i] = v;
 }
 
-__global__ void k_state_init(GmState* st, double eps)
+// Zeroes the state record (all threads of one block, 8 bytes each) and sets eps.
+static_assert(sizeof(GmState) % 8 == 0, "GmState is cleared in 8-byte words");
+__global__ void __launch_bounds__(kThreads) k_state_init(GmState* st, double eps)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        GmState z{};
-        z.eps = eps;
-        *st = z;
-    }
+    unsigned long long* w = reinterpret_cast<unsigned long long*>(st);
+    for (int i = threadIdx.x; i < (int)(sizeof(GmState) / 8); i += kThreads) w[i] = 0ull;
+    __syncthreads();
+    if (threadIdx.x == 0) st->eps = eps;
 }
 
 // r = b - r ; partial r . r
@@ -418,7 +419,7 @@ int gmres_solve(Mat& A, const double* b_in, double* x_out, const cel_gmres_param
     hstop[0] = hstop[1] = 0;
 
     CEL_HIP(hipEventRecord(e0, st));
-    k_state_init<<<1, 1, 0, st>>>(gs, prm.eps);
+    k_state_init<<<1, kThreads, 0, st>>>(gs, prm.eps);
     if (prm.x0_mode == CEL_X0_ONE || prm.x0_mode == CEL_X0_ZERO)
         k_fill<<<G, kThreads, 0, st>>>(n, ck, x, prm.x0_mode == CEL_X0_ONE ? 1.0 : 0.0);
     CEL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
