@@ -322,21 +322,50 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
             }
         }
     } else {
-        // kp right-hand sides: the kp lanes of a row read the row's entries
-        // (one broadcast per wave) and gather x[col][j] contiguously; no LDS,
-        // no barriers
+        // kp right-hand sides: the kp lanes of a row gather x[col][j]
+        // contiguously.  The block's steps (R = 256/kp rows each) go in
+        // chunks of kp steps = 256 rows whose row pointers and entries are
+        // staged in LDS with coalesced loads first, so a row's only global
+        // loads are its gathers (a row read straight from HBM waits on its
+        // row bounds, then its entries, then the gathers).  Rows are summed
+        // in entry order and added to the partial in step order; a chunk
+        // whose entries do not fit the stage reads them from HBM.
+        __shared__ int sx[kTileRows + 1];
         const int R = kBlock >> S.ksh;
         const int my = threadIdx.x >> S.ksh;
-        if (on) {
-            for (int t = blk.t0; t < blk.t1; ++t) {
+        for (int tc = blk.t0; tc < blk.t1; tc += S.kp) {
+            const int tn = min(tc + S.kp, blk.t1);
+            const int r0 = tc * R;
+            if (r0 >= S.n) break;  // uniform: steps past the last row hold no rows
+            const int r1 = min(tn * R, S.n);
+            const int e0 = S.xadj[r0];
+            const int cnt = S.xadj[r1] - e0;
+            const bool staged = cnt <= kStageCap;  // uniform across the block
+            __syncthreads();  // the previous chunk's stage is consumed
+            for (int q = threadIdx.x; q <= r1 - r0; q += kBlock) sx[q] = S.xadj[r0 + q] - e0;
+            if (staged)
+                for (int k = threadIdx.x; k < cnt; k += kBlock) {
+                    sm.val[k] = S.vals[e0 + k];
+                    sm.col[k] = S.cols[e0 + k];
+                }
+            __syncthreads();
+            if (!on) continue;
+            for (int t = tc; t < tn; ++t) {
                 const int row = t * R + my;
                 if (row < S.n) {
-                    const int lo = S.xadj[row], hi = S.xadj[row + 1];
+                    const int lo = sx[row - r0], hi = sx[row - r0 + 1];
                     double sum = 0.0;
+                    if (staged) {
 #pragma unroll 4
-                    for (int k = lo; k < hi; ++k) {
-                        double prod = S.vals[k] * ps[((size_t)S.cols[k] << S.ksh) + j];
-                        sum = sum + prod;
+                        for (int k = lo; k < hi; ++k) {
+                            double prod = sm.val[k] * ps[((size_t)sm.col[k] << S.ksh) + j];
+                            sum = sum + prod;
+                        }
+                    } else {
+                        for (int k = lo; k < hi; ++k) {
+                            double prod = S.vals[e0 + k] * ps[((size_t)S.cols[e0 + k] << S.ksh) + j];
+                            sum = sum + prod;
+                        }
                     }
                     const size_t e = ((size_t)row << S.ksh) + j;
                     const double vi = 0.0 + sum;
