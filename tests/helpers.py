@@ -110,3 +110,18 @@ def delta_apply(shape, kappa, kappa_p, f0, v):
             sl[ax] = end
             out[tuple(sl)] += dk[tuple(sl)] * v3[tuple(sl)]
     return out.ravel()
+
+
+def stencil27(nx, ny, nz):
+    """27 I - (all-ones 27-point stencil), Dirichlet outside, as a 0-based
+    scipy CSR matrix: up to 27 entries per row, symmetric and diagonally
+    dominant (SPD); 27 diagonals."""
+    import scipy.sparse as sp
+
+    def t(m):
+        return sp.diags([np.ones(m - 1), np.ones(m), np.ones(m - 1)], [-1, 0, 1])
+
+    K = sp.kron(t(nz), sp.kron(t(ny), t(nx)))
+    A = (27.0 * sp.identity(nx * ny * nz) - K).tocsr()
+    A.sort_indices()
+    return A

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from helpers import stencil27
 
 pytestmark = pytest.mark.gpu
 
@@ -212,3 +213,53 @@ def test_batch_rejects_mixed_precision(gpu):
     batch = gpu.Batch([M])
     with pytest.raises(gpu.CelError):
         batch.solve([np.ones(M.n)], precision=gpu.PREC_MIXED)
+
+
+def test_batch_rows_beyond_the_lds_stage(gpu):
+    """256-row chunks with more than 2048 entries (27-point rows) read their
+    entries from HBM instead of the LDS stage; a block-diagonal matrix of a
+    7-point and a 27-point part mixes staged and unstaged chunks in one
+    system.  One, 4 and 16 right-hand sides cover the one-rhs tile and the
+    multi-rhs chunk path; every lane is checked against the oracle."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(21)
+    P7 = O.gen_poisson7(8, 8, 8)
+    P = sp.csr_matrix((P7.vals, P7.cols - 1, P7.xadj - 1))
+    systems = [(sp.block_diag([P, stencil27(6, 7, 8)]).tocsr(), 4, True),
+               (stencil27(7, 6, 5), 1, True),
+               (sp.block_diag([stencil27(5, 6, 7), P]).tocsr(), 16, True),
+               (P, 3, False)]
+    mats, bs, nrhs, hosts = [], [], [], []
+    for A, k, dense in systems:
+        A.sort_indices()
+        per_row = np.diff(A.indptr)
+        most = max(per_row[i:i + 256].sum() for i in range(0, A.shape[0], 256))
+        assert (most > 2048) == dense, most
+        M = gpu.Matrix.from_csr(A.indptr + 1, A.indices + 1, A.data)
+        M.set_jacobi()
+        B = rng.uniform(-1.0, 1.0, (k, A.shape[0]))
+        B *= S / np.linalg.norm(B, axis=1, keepdims=True)
+        mats.append(M)
+        bs.append(B[0] if k == 1 else B)
+        nrhs.append(k)
+        hosts.append(O.CSR(A.indptr + 1, A.indices + 1, A.data))
+    out = gpu.Batch(mats, nrhs).solve(bs, tol=1e-9, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO,
+                                      alpha_stop=1)
+    assert out["rc"] == 0, out["total"]
+    xs = lanes_of(out["x"], nrhs)
+    lane = 0
+    for A, b, k in zip(hosts, bs, nrhs):
+        jac = O.set_jacobi(O.csr_diagonal(A))
+        for j in range(k):
+            bj = b if k == 1 else b[j]
+            ref = O.pcg_jacobi(A, jac, bj, x0_mode=O.X0_ZERO, eps=1e-9 * np.linalg.norm(bj),
+                               iter_max=10000, workers=3)
+            st = out["lanes"][lane]
+            assert st["status"] == 0
+            assert abs(st["iters"] - ref["iters"]) <= 1, (lane, st["iters"], ref["iters"])
+            if st["iters"] == ref["iters"]:
+                np.testing.assert_allclose(xs[lane], ref["x"], rtol=0,
+                                           atol=1e-10 * np.abs(ref["x"]).max())
+            lane += 1
+    assert lane == 4 + 1 + 16 + 3

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from helpers import stencil27
 
 pytestmark = pytest.mark.gpu
 
@@ -294,3 +295,42 @@ def test_spmv_block_without_columns(gpu):
     np.testing.assert_array_equal(M.spmv(np.zeros(0)), np.zeros(2))
     y0 = np.array([3.0, -1.0])
     np.testing.assert_array_equal(M.spmv(np.zeros(0), y0, accumulate=True), y0)
+
+
+@pytest.mark.parametrize("fmt", ["csr", "jad", "dia"])
+def test_rows_beyond_the_lds_stage_parity(gpu, fmt):
+    """27-point rows (up to 27 entries): every 256-row tile holds more than
+    the 2048 entries the K1 stages in LDS, so the CSR K1 reads them from HBM;
+    JAD has 27 diagonals and the DIA form 27 diagonals with 32-bit row masks.
+    FP64 against the oracle's history; DIA bitwise the CSR solve, FP64 and
+    mixed."""
+    A = stencil27(14, 13, 12)
+    per_row = np.diff(A.indptr)
+    assert min(per_row[i:i + 256].sum() for i in range(0, A.shape[0] - 255, 256)) > 2048
+    H = O.CSR(A.indptr + 1, A.indices + 1, A.data)
+    b = 1e3 * np.random.default_rng(31).uniform(-1.0, 1.0, H.n)
+    jac = O.set_jacobi(O.csr_diagonal(H))
+    ref = O.pcg_jacobi(H, jac, b, x0_mode=O.X0_ZERO, eps=1e-10 * np.linalg.norm(b), iter_max=2000,
+                       workers=3)
+    M = gpu.Matrix.from_csr(H.xadj, H.cols, H.vals)
+    M.set_jacobi()
+    kw = dict(tol=1e-10, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, iter_max=2000)
+    f = gpu.FMT_CSR
+    if fmt == "jad":
+        M.to_jad()
+        f = gpu.FMT_JAD
+    elif fmt == "dia":
+        assert len(M.to_dia()) == 27
+        f = gpu.FMT_DIA
+    got = M.pcg(b, fmt=f, history=True, **kw)
+    print(f"{fmt}: iters gpu {got['iters']} ref {ref['iters']}", flush=True)
+    assert got["status"] == 0
+    compare(ref, got)
+    np.testing.assert_allclose(got["x"], ref["x"], rtol=0, atol=1e-10 * np.abs(ref["x"]).max())
+    if fmt == "dia":
+        for prec in (gpu.PREC_FP64, gpu.PREC_MIXED):
+            a = M.pcg(b, precision=prec, **kw)
+            d = M.pcg(b, fmt=gpu.FMT_DIA, precision=prec, **kw)
+            assert a["status"] == d["status"] == 0
+            assert (a["iters"], a["outer_iters"]) == (d["iters"], d["outer_iters"])
+            np.testing.assert_array_equal(a["x"], d["x"])
