@@ -137,7 +137,10 @@ __device__ __forceinline__ void dia_rows(int n, int nd, const int* doff, const V
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
                     v[g][k] = buf_load_nt<VT>(rv, rc[k] * B, 0);
-                    xv[g][k] = buf_load<VT>(rx, (rc[k] + o) * B);
+                    // an absent entry's column may lie outside [0, n): its
+                    // byte offset wraps modulo 2^32 (unsigned, defined) and
+                    // the load returns 0 or a value the mask discards
+                    xv[g][k] = buf_load<VT>(rx, (int)((unsigned)(rc[k] + o) * (unsigned)B));
                 }
             }
         }
