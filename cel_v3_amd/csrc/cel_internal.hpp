@@ -214,6 +214,19 @@ int launch_jad_spmv_f32(const Mat& A, const float* x, float* y, int accumulate);
 int launch_dia_spmv(const Mat& A, const double* x, double* y, int accumulate,
                     const int* skip = nullptr);
 int launch_dia_spmv_f32(Mat& A, const float* x, float* y, int accumulate);
+// FP64 products fused with the dot product that follows them in the GMRES
+// loop, over G fixed row chunks of `chunk` rows (a multiple of 256): block b
+// covers rows [b*chunk, b*chunk + chunk) and writes part[b].
+//   kFuseDot:   y = A x,        part[b] = sum of aux_i * y_i over chunk b
+//   kFuseResid: y = aux - A x,  part[b] = sum of y_i * y_i over chunk b
+// y holds the plain product's values bit for bit, and each thread adds its
+// rows in ascending order before the block sum, so part[b] equals what a
+// separate chunked dot kernel forms over the same rows.
+enum { kFuseDot = 1, kFuseResid = 2 };
+int launch_csr_spmv_fused(const Mat& A, int fuse, long long chunk, int G, const double* x,
+                          double* y, const double* aux, double* part, const int* skip);
+int launch_dia_spmv_fused(const Mat& A, int fuse, long long chunk, int G, const double* x,
+                          double* y, const double* aux, double* part, const int* skip);
 int dia_prepare_f32(Mat& A);
 // Sets bit `bit` (= dia_nd) of the DIA row masks on the rows with halo
 // entries and clears it elsewhere; fails when the masks have no spare bit.

@@ -131,6 +131,42 @@ int launch_dia(const Mat& A, const VT* vals, const VT* x, VT* y, int accumulate,
     return CEL_OK;
 }
 
+// FP64 product fused with the next dot product (kFuseDot / kFuseResid,
+// cel_internal.hpp): block b covers the rows of its chunk [b*chunk,
+// b*chunk + chunk) in steps of R*256 rows, R rows 256 apart per thread (the
+// plain FP64 shape), so each thread meets its rows in ascending order.
+template <int FUSE>
+__global__ void __launch_bounds__(kBlock)
+k_dia_spmv_fused(int n, long long chunk, Offsets O, const double* __restrict__ dv,
+                 const uint32_t* __restrict__ mask, const uint8_t* __restrict__ mask8,
+                 const double* __restrict__ x, double* __restrict__ y,
+                 const double* __restrict__ aux, double* __restrict__ part,
+                 const int* __restrict__ skip)
+{
+    if (skip && *skip) return;  // uniform across the grid
+    __shared__ double red[kWaves];
+    constexpr int R = DiaShape<double>::R, GD = DiaShape<double>::GD;
+    const long long c0 = (long long)blockIdx.x * chunk;
+    const long long c1 = c0 + chunk < n ? c0 + chunk : n;
+    double acc = 0.0;
+    for (long long s0 = c0; s0 < c1; s0 += (long long)R * kBlock) {
+        int r[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const long long rk = s0 + threadIdx.x + (long long)k * kBlock;
+            r[k] = rk < c1 ? (int)rk : n;  // rows past the chunk are not used
+        }
+        if (r[0] >= n) continue;  // no barrier in this loop
+        double s[R];
+        dia_rows<double, R, GD>(n, O.nd, O.off, dv, mask8, mask, r, x, s);
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (r[k] < n) acc = acc + fuse_row<FUSE>(s[k], r[k], aux, y);
+    }
+    const double bs = block_sum(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
 
 // Halo-row flag (bit `bit` of the row masks): cleared on every row, then
 // set on the rows that have halo entries.
@@ -178,6 +214,27 @@ int launch_dia_spmv(const Mat& A, const double* x, double* y, int accumulate, co
 {
     if (A.n == 0) return CEL_OK;
     return launch_dia<double>(A, A.dia_vals.p, x, y, accumulate, skip);
+}
+
+int launch_dia_spmv_fused(const Mat& A, int fuse, long long chunk, int G, const double* x,
+                          double* y, const double* aux, double* part, const int* skip)
+{
+    if (A.n == 0 || A.dia_nd == 0 || A.n != A.ncols)
+        return fail(CEL_ERR_ARG, "fused product: a square block with a DIA form");
+    if (chunk <= 0 || chunk % kBlock != 0 || (long long)G * chunk < A.n)
+        return fail(CEL_ERR_ARG, "fused product: chunks must be whole tiles covering every row");
+    if (fuse == kFuseDot)
+        hipLaunchKernelGGL(k_dia_spmv_fused<kFuseDot>, dim3(G), dim3(kBlock), 0, rt().stream, A.n,
+                           chunk, offsets_of(A), A.dia_vals.p, A.dia_mask.p, A.dia_mask8.p, x, y,
+                           aux, part, skip);
+    else if (fuse == kFuseResid)
+        hipLaunchKernelGGL(k_dia_spmv_fused<kFuseResid>, dim3(G), dim3(kBlock), 0, rt().stream, A.n,
+                           chunk, offsets_of(A), A.dia_vals.p, A.dia_mask.p, A.dia_mask8.p, x, y,
+                           aux, part, skip);
+    else
+        return fail(CEL_ERR_ARG, "fused product: unknown epilogue");
+    CEL_HIP(hipGetLastError());
+    return CEL_OK;
 }
 
 int dia_prepare_f32(Mat& A)

@@ -24,7 +24,11 @@
 // Each vector kernel writes per-block partial sums over fixed contiguous
 // chunks; the next kernel sums them in a fixed order (every block that needs
 // the value, or a one-block kernel that updates the state), so the result
-// depends on n only, never on timing.  The small Hessenberg algebra
+// depends on n only, never on timing.  In the CSR and DIA forms the products
+// w = A z_j and r = b - A x form the partials of v_1 . w and r . r
+// themselves, over the same chunks and in the same order as the separate
+// dot kernels the JAD form runs after its product, so all three forms give
+// the same bits.  The small Hessenberg algebra
 // (rotations, convergence test, back substitution) runs in one thread of a
 // one-block kernel, in the reference's order.  Partials alternate between
 // two buffers, so no kernel writes the partials it reads, and every state
@@ -317,17 +321,36 @@ struct Gmres {
     Gmres(const Mat& M, int fmt) : A(M), format(fmt), n(M.n), st(rt().stream)
     {
         // fixed chunks of a multiple of the workgroup size: the grouping of
-        // every dot product depends on n only
-        long long g = (n + 4095) / 4096;
+        // every dot product depends on n only (2048 rows per chunk up to
+        // 1024 chunks: one 8-row step per thread of the fused DIA product)
+        long long g = (n + 2047) / 2048;
         G = (int)(g < 1 ? 1 : g > 1024 ? 1024 : g);
         chunk = ((n + G - 1) / G + kThreads - 1) / kThreads * kThreads;
         if (chunk < kThreads) chunk = kThreads;
     }
-    int spmv(const double* x, double* y, const int* skip)
+    // w = A z with the partials of v1 . w (skipped once halt is set)
+    int spmv_dot(const double* z, double* w, const double* v1, double* part, GmState* gs)
     {
-        if (format == CEL_FMT_JAD) return launch_jad_spmv(A, x, y, 0, skip);
-        if (format == CEL_FMT_DIA) return launch_dia_spmv(A, x, y, 0, skip);
-        return launch_csr_spmv(A, x, y, 0, skip);
+        const int* skip = &gs->halt;
+        if (format == CEL_FMT_DIA)
+            return launch_dia_spmv_fused(A, kFuseDot, chunk, G, z, w, v1, part, skip);
+        if (format == CEL_FMT_CSR)
+            return launch_csr_spmv_fused(A, kFuseDot, chunk, G, z, w, v1, part, skip);
+        CEL_TRY(launch_jad_spmv(A, z, w, 0, skip));
+        k_dot<<<G, kThreads, 0, st>>>(n, chunk, v1, w, part, gs);
+        return CEL_OK;
+    }
+    // r = b - A x with the partials of r . r (skipped once stop is set)
+    int spmv_resid(const double* x, double* r, const double* b, double* part, GmState* gs)
+    {
+        const int* skip = &gs->stop;
+        if (format == CEL_FMT_DIA)
+            return launch_dia_spmv_fused(A, kFuseResid, chunk, G, x, r, b, part, skip);
+        if (format == CEL_FMT_CSR)
+            return launch_csr_spmv_fused(A, kFuseResid, chunk, G, x, r, b, part, skip);
+        CEL_TRY(launch_jad_spmv(A, x, r, 0, skip));
+        k_resid<<<G, kThreads, 0, st>>>(n, chunk, b, r, part, gs);
+        return CEL_OK;
     }
 };
 
@@ -375,23 +398,20 @@ int gmres_solve(Mat& A, const double* b_in, double* x_out, const cel_gmres_param
         x = xb.p;
     }
     GmState* gs = state.p;
-    const int* halt = &gs->halt;
     const int* stop = &gs->stop;
     auto vv = [&](int k) { return V.p + (size_t)(k - 1) * n; };  // v_k, 1-based
     auto zz = [&](int k) { return Z.p + (size_t)(k - 1) * n; };
 
     // one outer iteration: r = b - A x, then up to m+1 cycles
     auto enqueue_outer = [&]() -> int {
-        CEL_TRY(S.spmv(x, r.p, stop));
-        k_resid<<<G, kThreads, 0, st>>>(n, ck, b, r.p, partA.p, gs);
+        CEL_TRY(S.spmv_resid(x, r.p, b, partA.p, gs));
         k_outer_start<<<1, kThreads, 0, st>>>(partA.p, G, gs);
         for (int c = 0; c <= m; ++c) {
             k_scale_jac<<<G, kThreads, 0, st>>>(n, ck, r.p, A.jac.p, vv(1), zz(1), gs);
             for (int jj = 1; jj <= m; ++jj) {
-                CEL_TRY(S.spmv(zz(jj), w.p, halt));
                 double* cur = partA.p;
                 double* nxt = partB.p;
-                k_dot<<<G, kThreads, 0, st>>>(n, ck, vv(1), w.p, cur, gs);
+                CEL_TRY(S.spmv_dot(zz(jj), w.p, vv(1), cur, gs));
                 for (int ii = 1; ii <= jj; ++ii) {
                     const double* next = ii < jj ? vv(ii + 1) : nullptr;
                     k_mgs<<<G, kThreads, 0, st>>>(n, ck, ii, jj, cur, G, vv(ii), w.p, next, nxt, gs);
@@ -400,8 +420,7 @@ int gmres_solve(Mat& A, const double* b_in, double* x_out, const cel_gmres_param
                 k_givens<<<1, kThreads, 0, st>>>(jj, m, cur, G, gs);
                 k_next<<<G, kThreads, 0, st>>>(n, ck, jj, m, w.p, A.jac.p, V.p, Z.p, x, gs);
             }
-            CEL_TRY(S.spmv(x, r.p, stop));
-            k_resid<<<G, kThreads, 0, st>>>(n, ck, b, r.p, partA.p, gs);
+            CEL_TRY(S.spmv_resid(x, r.p, b, partA.p, gs));
             k_cycle_end<<<1, kThreads, 0, st>>>(partA.p, G, gs);
         }
         CEL_HIP(hipGetLastError());

@@ -39,6 +39,50 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmv(int n, const int* __restric
     }
 }
 
+// FP64 product fused with the next dot product (kFuseDot / kFuseResid,
+// cel_internal.hpp): block b walks the 256-row tiles of its row chunk
+// [b*chunk, b*chunk + chunk) with the same tile stream as k_csr_spmv (same
+// row sums), each thread its rows in ascending order.
+template <int FUSE>
+__global__ __launch_bounds__(kBlock) void k_csr_spmv_fused(int n, long long chunk,
+                                                           const int* __restrict__ xadj,
+                                                           const int* __restrict__ cols,
+                                                           const double* __restrict__ vals,
+                                                           const double* __restrict__ x,
+                                                           double* __restrict__ y,
+                                                           const double* __restrict__ aux,
+                                                           double* __restrict__ part,
+                                                           const int* __restrict__ skip)
+{
+    if (skip && *skip) return;  // uniform across the grid
+    __shared__ CsrStage<double> sm;
+    __shared__ double red[kWaves];
+    const long long c0 = (long long)blockIdx.x * chunk;
+    const long long c1 = c0 + chunk < n ? c0 + chunk : n;
+    const int t0 = (int)(c0 / kTileRows);  // chunk is a multiple of the tile
+    const int t1 = c0 < c1 ? (int)((c1 + kTileRows - 1) / kTileRows) : t0;
+    CsrTileStream<double> ts;
+    if (t0 < t1) {  // uniform across the block
+        ts.fetch(n, t0, true, xadj, cols, vals);
+        ts.stage(sm);
+        __syncthreads();
+    }
+    double acc = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        const int row = t * kTileRows + threadIdx.x;
+        const int nxt = t + 1 < t1 ? t + 1 : -1;
+        double s = ts.template rows<double>(n, t, nxt, xadj, cols, vals, x, sm, row, nullptr);
+        __syncthreads();
+        if (nxt >= 0) {
+            ts.stage(sm);
+            __syncthreads();
+        }
+        if (row < n) acc = acc + fuse_row<FUSE>(0.0 + s, row, aux, y);
+    }
+    const double bs = block_sum(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_jad_spmv(int n, int mmax, const int* __restrict__ begin,
                                                      const int* __restrict__ jcols,
@@ -79,6 +123,25 @@ int launch_csr_spmv(const Mat& A, const double* x, double* y, int accumulate, co
     int G = resident_grid((const void*)k_csr_spmv<double>, ntiles);
     hipLaunchKernelGGL(k_csr_spmv<double>, dim3(G), dim3(kBlock), 0, rt().stream, A.n, A.xadj.p,
                        A.cols.p, A.vals.p, x, y, accumulate, ntiles, tiles_per_block(ntiles, G), skip);
+    CEL_HIP(hipGetLastError());
+    return CEL_OK;
+}
+
+int launch_csr_spmv_fused(const Mat& A, int fuse, long long chunk, int G, const double* x,
+                          double* y, const double* aux, double* part, const int* skip)
+{
+    if (A.n == 0 || A.ncols == 0 || A.n != A.ncols)
+        return fail(CEL_ERR_ARG, "fused product: a square block with rows");
+    if (chunk <= 0 || chunk % kTileRows != 0 || (long long)G * chunk < A.n)
+        return fail(CEL_ERR_ARG, "fused product: chunks must be whole tiles covering every row");
+    if (fuse == kFuseDot)
+        hipLaunchKernelGGL(k_csr_spmv_fused<kFuseDot>, dim3(G), dim3(kBlock), 0, rt().stream, A.n,
+                           chunk, A.xadj.p, A.cols.p, A.vals.p, x, y, aux, part, skip);
+    else if (fuse == kFuseResid)
+        hipLaunchKernelGGL(k_csr_spmv_fused<kFuseResid>, dim3(G), dim3(kBlock), 0, rt().stream, A.n,
+                           chunk, A.xadj.p, A.cols.p, A.vals.p, x, y, aux, part, skip);
+    else
+        return fail(CEL_ERR_ARG, "fused product: unknown epilogue");
     CEL_HIP(hipGetLastError());
     return CEL_OK;
 }

@@ -23,6 +23,7 @@
 // bit-identical to the CSR sum.
 #pragma once
 
+#include "cel_internal.hpp"
 #include "device_common.hpp"
 
 namespace cel {
@@ -34,6 +35,25 @@ constexpr int kPer = kStageCap / kBlock;   // staged entries per thread
 // up to 3 entries before it (at a 16-byte boundary)
 constexpr int kStageMax = kStageCap - 4;
 constexpr int kGather = 8;                 // x values gathered per round trip
+
+// Epilogue of a fused FP64 product row (kFuseDot / kFuseResid): stores the
+// row's value (s = the plain product's value) and returns the row's term of
+// the block's dot-product partial.
+template <int FUSE>
+__device__ __forceinline__ double fuse_row(double s, long long row, const double* __restrict__ aux,
+                                           double* __restrict__ y)
+{
+    if constexpr (FUSE == kFuseDot) {
+        y[row] = s;
+        const double p = aux[row] * s;
+        return p;
+    } else {
+        const double ri = aux[row] - s;  // r = b - A x
+        y[row] = ri;
+        const double p = ri * ri;
+        return p;
+    }
+}
 
 template <typename VT>
 struct CsrStage {

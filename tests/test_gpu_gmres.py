@@ -67,6 +67,36 @@ def test_gmres_device_arrays_jad_and_repeat_are_bitwise(gpu):
     assert j["iters"] == a["iters"] and j["cycles"] == a["cycles"]
 
 
+@pytest.mark.parametrize("shape", [(10, 10, 10), (24, 22, 20)])
+def test_gmres_csr_jad_dia_are_bitwise_and_match_oracle(gpu, shape):
+    """The CSR and DIA products form the partials of v_1 . w and r . r
+    themselves, over the GMRES row chunks; JAD runs its product and a
+    separate dot kernel over the same chunks: the three forms give the same
+    bits.  24x22x20: 10560 rows in 6 chunks of 1792 (the last one partial),
+    so a fused DIA step (2048 rows) covers rows of the next chunk that it
+    must leave out."""
+    A = convdiff_csr(*shape)
+    b = np.random.default_rng(57).random(A.n)
+    M = dev(gpu, A)
+    kw = dict(eps=1e-8, restart=3, iter_max=5000)
+    c = M.gmres(b, **kw)
+    M.to_jad()
+    j = M.gmres(b, fmt=gpu.FMT_JAD, **kw)
+    M.to_dia()
+    d = M.gmres(b, fmt=gpu.FMT_DIA, **kw)
+    for o in (j, d):
+        assert (o["status"], o["iters"], o["cycles"]) == (c["status"], c["iters"], c["cycles"])
+        np.testing.assert_array_equal(o["x"], c["x"])
+        assert o["final_norm"] == c["final_norm"]
+    jac = O.set_jacobi(O.csr_diagonal(A))
+    ref = O.gmres_jacobi(A, jac, b, eps=1e-8, m=3, iter_max=5000)
+    print(f"{shape}: gpu {c['iters']}/{c['cycles']} oracle {ref['iters']}/{ref['cycles']}", flush=True)
+    assert c["status"] == 0 and ref["reason"] == 2
+    assert abs(c["iters"] - ref["iters"]) <= 1 and abs(c["cycles"] - ref["cycles"]) <= 4
+    if c["cycles"] == ref["cycles"]:
+        np.testing.assert_allclose(c["x"], ref["x"], rtol=0, atol=1e-10 * np.abs(ref["x"]).max())
+
+
 def test_gmres_iteration_cap_and_errors(gpu):
     A = O.gen_poisson7(12, 12, 12)
     b = np.random.default_rng(54).random(A.n)
