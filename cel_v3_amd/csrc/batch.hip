@@ -342,11 +342,13 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
             const int cnt = S.xadj[r1] - e0;
             const bool staged = cnt <= kStageCap;  // uniform across the block
             __syncthreads();  // the previous chunk's stage is consumed
-            for (int q = threadIdx.x; q <= r1 - r0; q += kBlock) sx[q] = S.xadj[r0 + q] - e0;
+            // the matrix is read once per product: non-temporal loads keep
+            // it from evicting the gathered x lines from L2
+            for (int q = threadIdx.x; q <= r1 - r0; q += kBlock) sx[q] = ld_stream(S.xadj + r0 + q) - e0;
             if (staged)
                 for (int k = threadIdx.x; k < cnt; k += kBlock) {
-                    sm.val[k] = S.vals[e0 + k];
-                    sm.col[k] = S.cols[e0 + k];
+                    sm.val[k] = ld_stream(S.vals + e0 + k);
+                    sm.col[k] = ld_stream(S.cols + e0 + k);
                 }
             __syncthreads();
             if (!on) continue;
@@ -363,13 +365,14 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
                         }
                     } else {
                         for (int k = lo; k < hi; ++k) {
-                            double prod = S.vals[e0 + k] * ps[((size_t)S.cols[e0 + k] << S.ksh) + j];
+                            double prod = ld_stream(S.vals + e0 + k) *
+                                          ps[((size_t)ld_stream(S.cols + e0 + k) << S.ksh) + j];
                             sum = sum + prod;
                         }
                     }
                     const size_t e = ((size_t)row << S.ksh) + j;
                     const double vi = 0.0 + sum;
-                    vs[e] = vi;
+                    __builtin_nontemporal_store(vi, vs + e);  // read by the next kernel only
                     double prod = vi * ps[e];
                     acc = acc + prod;
                 }
