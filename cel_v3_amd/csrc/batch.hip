@@ -333,8 +333,15 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
         __shared__ int sx[kTileRows + 1];
         const int R = kBlock >> S.ksh;
         const int my = threadIdx.x >> S.ksh;
-        for (int tc = blk.t0; tc < blk.t1; tc += S.kp) {
-            const int tn = min(tc + S.kp, blk.t1);
+        // chunks are dealt round-robin over the system's blocks (chunk c to
+        // block b0 + c mod nb), so the blocks sweep the rows together and
+        // rows a band apart are gathered close in time
+        const int nb = S.b1 - S.b0;
+        const int first = b - S.b0;  // the table keeps b in [b0, b1)
+        const int nchunks = (first >= 0 && first < nb) ? (S.ntiles + S.kp - 1) / S.kp : 0;
+        for (int c = first; c < nchunks; c += nb) {
+            const int tc = c * S.kp;
+            const int tn = min(tc + S.kp, S.ntiles);
             const int r0 = tc * R;
             if (r0 >= S.n) break;  // uniform: steps past the last row hold no rows
             const int r1 = min(tn * R, S.n);
