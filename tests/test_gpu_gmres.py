@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from helpers import stencil27
 from test_oracle import convdiff_csr
 
 pytestmark = pytest.mark.gpu
@@ -67,15 +68,22 @@ def test_gmres_device_arrays_jad_and_repeat_are_bitwise(gpu):
     assert j["iters"] == a["iters"] and j["cycles"] == a["cycles"]
 
 
-@pytest.mark.parametrize("shape", [(10, 10, 10), (24, 22, 20)])
-def test_gmres_csr_jad_dia_are_bitwise_and_match_oracle(gpu, shape):
+@pytest.mark.parametrize("kind,shape", [("convdiff", (10, 10, 10)), ("convdiff", (24, 22, 20)),
+                                        ("stencil27", (14, 13, 12))])
+def test_gmres_csr_jad_dia_are_bitwise_and_match_oracle(gpu, kind, shape):
     """The CSR and DIA products form the partials of v_1 . w and r . r
     themselves, over the GMRES row chunks; JAD runs its product and a
     separate dot kernel over the same chunks: the three forms give the same
     bits.  24x22x20: 10560 rows in 6 chunks of 1792 (the last one partial),
     so a fused DIA step (2048 rows) covers rows of the next chunk that it
-    must leave out."""
-    A = convdiff_csr(*shape)
+    must leave out.  The 27-point matrix (2184 rows, 2 chunks) has tiles of
+    more than 2048 entries (the fused CSR product reads them from HBM) and
+    27 diagonals (32-bit DIA row masks)."""
+    if kind == "stencil27":
+        S27 = stencil27(*shape)
+        A = O.CSR(S27.indptr + 1, S27.indices + 1, S27.data)
+    else:
+        A = convdiff_csr(*shape)
     b = np.random.default_rng(57).random(A.n)
     M = dev(gpu, A)
     kw = dict(eps=1e-8, restart=3, iter_max=5000)
