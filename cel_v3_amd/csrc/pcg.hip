@@ -98,6 +98,28 @@ __device__ __forceinline__ double all_sum(const double* __restrict__ part, int G
     return v;
 }
 
+// Two folds at once, each in all_sum's order (thread-strided sums, then the
+// block tree of block_sum): one pass over both partial arrays.
+__device__ __forceinline__ void all_sum2(const double* __restrict__ pa, const double* __restrict__ pb,
+                                         int G, double* red /* 2*kWaves */, double* bcast /* 2 */,
+                                         double& a, double& b)
+{
+    double sa = 0.0, sb = 0.0;
+    for (int i = threadIdx.x; i < G; i += kBlock) {
+        sa = sa + __builtin_nontemporal_load(pa + i);
+        sb = sb + __builtin_nontemporal_load(pb + i);
+    }
+    block_sum2(sa, sb, red);
+    if (threadIdx.x == 0) {
+        bcast[0] = sa;
+        bcast[1] = sb;
+    }
+    __syncthreads();
+    a = bcast[0];
+    b = bcast[1];
+    __syncthreads();
+}
+
 // Balanced contiguous ranges: logical block lb of G owns units [u0, u1).
 __device__ __forceinline__ void unit_range(long long nunits, int G, int lb, long long& u0,
                                            long long& u1)
@@ -588,15 +610,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xp(long long n, XT* __res
                                                           const Ctrl* __restrict__ cin,
                                                           Ctrl* __restrict__ cout, Prm P)
 {
-    __shared__ double red[kWaves];
-    __shared__ double bcast;
+    __shared__ double red[2 * kWaves];
+    __shared__ double bcast[2];
     Ctrl c = read_ctrl(cin);
     if (!c.run) {
         write_ctrl(cout, c);
         return;
     }
-    const double rz = all_sum(rpart, G2, red, &bcast);
-    const double rr = all_sum(rpart + G2, G2, red, &bcast);
+    double rz, rr;
+    all_sum2(rpart, rpart + G2, G2, red, bcast, rz, rr);  // r.z and r.r in one pass
     const double beta_d = rz / c.alpha;
     const VT sigma = VT(c.sigma), beta = VT(beta_d);
     // scalars of the next iteration (they do not depend on the vectors)
