@@ -312,13 +312,13 @@ __global__ __launch_bounds__(kBlock) void k_residual(MatView A, const double* __
     write_ctrl(cout, c);
 }
 
-// Tile order of the CSR and DIA K1: tiles go in groups of kK1Group
+// Tile order of the CSR, DIA and JAD K1: tiles go in groups of kK1Group
 // consecutive tiles, group q to block q mod G; step k of a block covers
 // tile kK1Group*q + k mod kK1Group of its (k / kK1Group)-th group
 // (-1: past the end).  Both forms add a thread's rows to the block partial
 // in this order, so their partials and solves are bit-identical;
 // neighbouring blocks stream neighbouring groups (measured faster on the
-// 256^3 solve than a contiguous tile range per block, for both forms).
+// 256^3 solve than a contiguous tile range per block, in all three forms).
 // (groups of 4 measured: FP64 DIA solve 1% faster, mixed DIA 1% slower)
 constexpr int kK1Group = 2;
 __device__ __forceinline__ long long group_tile(long long k, long long ntiles)
@@ -464,14 +464,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
             }
         }
     } else {
-        long long t0, t1;
-        unit_range(ntiles, gridDim.x, lb, t0, t1);
+        // JAD: the CSR K1's tile-pair order (one permuted row per thread
+        // and tile; the tile walk needs no staging)
         TileWalk<VT, VT, FMT, false> walk;
-        walk.start(A, t0, t1, sm);
-        for (long long t = t0; t < t1; ++t) {
+        long long k = 0;
+        for (long long t = group_tile(0, ntiles); t >= 0; t = group_tile(++k, ntiles)) {
             int row;
             VT pr = VT(0);
-            VT s = walk.next(A, t, t1, p, sm, row, &pr);
+            VT s = walk.next(A, t, t + 1, p, sm, row, &pr);
             if (row < A.n) {
                 v[row] = s;
                 double prod = double(s) * double(pr);
