@@ -312,22 +312,10 @@ __global__ __launch_bounds__(kBlock) void k_residual(MatView A, const double* __
     write_ctrl(cout, c);
 }
 
-// Tile order of the CSR, DIA and JAD K1: tiles go in groups of kK1Group
-// consecutive tiles, group q to block q mod G; step k of a block covers
-// tile kK1Group*q + k mod kK1Group of its (k / kK1Group)-th group
-// (-1: past the end).  Both forms add a thread's rows to the block partial
-// in this order, so their partials and solves are bit-identical;
-// neighbouring blocks stream neighbouring groups (measured faster on the
-// 256^3 solve than a contiguous tile range per block, in all three forms).
-// (groups of 4 measured: FP64 DIA solve 1% faster, mixed DIA 1% slower)
-constexpr int kK1Group = 2;
-__device__ __forceinline__ long long group_tile(long long k, long long ntiles)
-{
-    const long long ngroups = (ntiles + kK1Group - 1) / kK1Group;
-    const long long q = (long long)blockIdx.x + (k / kK1Group) * gridDim.x;
-    const long long t = kK1Group * q + k % kK1Group;
-    return (q < ngroups && t < ntiles) ? t : -1;
-}
+// The CSR, DIA and JAD K1s walk their tiles in group_tile's order
+// (spmv_device.hpp); all forms add a thread's rows to the block partial in
+// that order, so their partials and solves are bit-identical.  (Groups of
+// 4 tiles measured: FP64 DIA solve 1% faster, mixed DIA 1% slower.)
 
 // K1: v = A p, per-block partial of v.p.  Folds pending set-up or
 // replacement partials into the token first (every block, same order) and
@@ -425,7 +413,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
     } else if constexpr (FMT == CEL_FMT_DIA) {
         // all tiles of a group per step, one row of each per thread; the
         // rows are added to the partial in tile order, as in the CSR K1
-        constexpr int R = kK1Group;
+        constexpr int R = kTileGroup;
         const long long ngroups = (ntiles + R - 1) / R;
         for (long long q = blockIdx.x; q < ngroups; q += gridDim.x) {
             int r[R];

@@ -4,31 +4,28 @@
 
 namespace cel {
 
-// ------------------------------------------------------------------ grid sizing
-static int tiles_per_block(long long tiles, int G) { return (int)((tiles + G - 1) / G); }
-
 // ------------------------------------------------------------------ CSR / JAD SpMV
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_csr_spmv(int n, const int* __restrict__ xadj,
                                                      const int* __restrict__ cols,
                                                      const VT* __restrict__ vals,
                                                      const VT* __restrict__ x, VT* __restrict__ y,
-                                                     int accumulate, int ntiles, int tpb,
+                                                     int accumulate, int ntiles,
                                                      const int* __restrict__ skip)
 {
     if (skip && *skip) return;  // uniform across the grid
     __shared__ CsrStage<VT> sm;
-    const int lb = logical_block();
-    const int t0 = lb * tpb, t1 = min(t0 + tpb, ntiles);
     CsrTileStream<VT> ts;
-    if (t0 < t1) {  // uniform across the block
-        ts.fetch(n, t0, true, xadj, cols, vals);
+    const int tfirst = (int)group_tile(0, ntiles);
+    if (tfirst >= 0) {  // uniform across the block
+        ts.fetch(n, tfirst, true, xadj, cols, vals);
         ts.stage(sm);
         __syncthreads();
     }
-    for (int t = t0; t < t1; ++t) {
+    int step = 0;
+    for (int t = tfirst; t >= 0; t = (int)group_tile(++step, ntiles)) {
         const int row = t * kTileRows + threadIdx.x;
-        const int nxt = t + 1 < t1 ? t + 1 : -1;
+        const int nxt = (int)group_tile(step + 1, ntiles);
         VT s = ts.template rows<VT>(n, t, nxt, xadj, cols, vals, x, sm, row, nullptr);
         __syncthreads();
         if (nxt >= 0) {
@@ -89,13 +86,12 @@ __global__ __launch_bounds__(kBlock) void k_jad_spmv(int n, int mmax, const int*
                                                      const VT* __restrict__ jvals,
                                                      const int* __restrict__ perm,
                                                      const VT* __restrict__ x, VT* __restrict__ y,
-                                                     int accumulate, int ntiles, int tpb,
+                                                     int accumulate, int ntiles,
                                                      const int* __restrict__ skip)
 {
     if (skip && *skip) return;
-    const int lb = logical_block();
-    const int t0 = lb * tpb, t1 = min(t0 + tpb, ntiles);
-    for (int t = t0; t < t1; ++t) {
+    int step = 0;
+    for (int t = (int)group_tile(0, ntiles); t >= 0; t = (int)group_tile(++step, ntiles)) {
         const int k = t * kTileRows + threadIdx.x;
         if (k < n) {
             VT s = jad_row<VT, VT>(k, mmax, begin, jcols, jvals, x);
@@ -122,7 +118,7 @@ int launch_csr_spmv(const Mat& A, const double* x, double* y, int accumulate, co
     int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int G = resident_grid((const void*)k_csr_spmv<double>, ntiles);
     hipLaunchKernelGGL(k_csr_spmv<double>, dim3(G), dim3(kBlock), 0, rt().stream, A.n, A.xadj.p,
-                       A.cols.p, A.vals.p, x, y, accumulate, ntiles, tiles_per_block(ntiles, G), skip);
+                       A.cols.p, A.vals.p, x, y, accumulate, ntiles, skip);
     CEL_HIP(hipGetLastError());
     return CEL_OK;
 }
@@ -154,7 +150,7 @@ int launch_csr_spmv_f32(const Mat& A, const float* x, float* y, int accumulate)
     int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int G = resident_grid((const void*)k_csr_spmv<float>, ntiles);
     hipLaunchKernelGGL(k_csr_spmv<float>, dim3(G), dim3(kBlock), 0, rt().stream, A.n, A.xadj.p,
-                       A.cols.p, A.vals32.p, x, y, accumulate, ntiles, tiles_per_block(ntiles, G),
+                       A.cols.p, A.vals32.p, x, y, accumulate, ntiles,
                        nullptr);
     CEL_HIP(hipGetLastError());
     return CEL_OK;
@@ -167,7 +163,7 @@ int launch_jad_spmv(const Mat& A, const double* x, double* y, int accumulate, co
     int G = resident_grid((const void*)k_jad_spmv<double>, ntiles);
     hipLaunchKernelGGL(k_jad_spmv<double>, dim3(G), dim3(kBlock), 0, rt().stream, A.n, A.jad_mmax,
                        A.jad_begin.p, A.jad_cols.p, A.jad_vals.p, A.jad_perm.p, x, y, accumulate,
-                       ntiles, tiles_per_block(ntiles, G), skip);
+                       ntiles, skip);
     CEL_HIP(hipGetLastError());
     return CEL_OK;
 }
@@ -180,7 +176,7 @@ int launch_jad_spmv_f32(const Mat& A, const float* x, float* y, int accumulate)
     int G = resident_grid((const void*)k_jad_spmv<float>, ntiles);
     hipLaunchKernelGGL(k_jad_spmv<float>, dim3(G), dim3(kBlock), 0, rt().stream, A.n, A.jad_mmax,
                        A.jad_begin.p, A.jad_cols.p, A.jad_vals32.p, A.jad_perm.p, x, y, accumulate,
-                       ntiles, tiles_per_block(ntiles, G), nullptr);
+                       ntiles, nullptr);
     CEL_HIP(hipGetLastError());
     return CEL_OK;
 }
