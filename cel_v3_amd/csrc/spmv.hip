@@ -90,8 +90,10 @@ __global__ __launch_bounds__(kBlock) void k_jad_spmv(int n, int mmax, const int*
                                                      const int* __restrict__ skip)
 {
     if (skip && *skip) return;
-    int step = 0;
-    for (int t = (int)group_tile(0, ntiles); t >= 0; t = (int)group_tile(++step, ntiles)) {
+    // single tiles dealt round-robin (tile t to block t mod G): for JAD,
+    // whose tiles need no staging, measured faster than tile pairs (FP64
+    // 346 against 360-366 us on the 256^3 block)
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int k = t * kTileRows + threadIdx.x;
         if (k < n) {
             VT s = jad_row<VT, VT>(k, mmax, begin, jcols, jvals, x);

@@ -37,12 +37,12 @@ constexpr int kStageMax = kStageCap - 4;
 constexpr int kGather = 8;                 // x values gathered per round trip
 
 // Tile order of the kernels that walk whole 256-row tiles (the plain CSR
-// and JAD products, the PCG K1s): tiles go in groups of kTileGroup
+// product, the PCG K1s): tiles go in groups of kTileGroup
 // consecutive tiles, group q to block q mod G; step k of a block covers
 // tile kTileGroup*q + k mod kTileGroup of its (k / kTileGroup)-th group
 // (-1: past the end).  Neighbouring blocks stream neighbouring groups:
 // measured faster on the 256^3 7-point block than a contiguous tile range
-// per block (PCG K1s 3-7%, plain CSR 7-8%, plain JAD 12-14%).
+// per block (PCG K1s 3-7%, plain CSR 7-8%).
 constexpr int kTileGroup = 2;
 __device__ __forceinline__ long long group_tile(long long k, long long ntiles)
 {
