@@ -270,30 +270,25 @@ def sort_bench(cel, N, steps, warmup, skip_cpu):
                                             np.array_equal(c_s, cols[np.argsort(perm)]))
     if not skip_cpu:
         out["cpu_baseline"] = sort_cpu_baseline()
-        log(f"cpu baseline (reference sort): {out['cpu_baseline']}")
+        log(f"cpu baseline (oracle sort): {out['cpu_baseline']}")
     return out
 
 
 def sort_cpu_baseline():
-    """The reference's own cel_sp_mat_format_sparse_31_sort (oracle/_ref,
-    built from its sources; one thread, as it runs) on a bounded sample: the
-    160^3 7-point matrix (28,518,400 entries) in the same shuffled form, about
-    15 s of CPU work.  The oracle's restatement is timed instead when the
-    reference build is absent."""
+    """The oracle's restatement of the reference's COO sort
+    (orc_coo_sort_cel: cel_sp_mat_format_sparse_31_sort, one thread, as the
+    reference runs it) on a bounded sample: the 160^3 7-point matrix
+    (28,518,400 entries) in the same shuffled form, about 15 s of CPU work."""
     import oracle as O
 
     A = O.gen_poisson7(160, 160, 160)
     rows = np.repeat(np.arange(1, A.n + 1, dtype=np.int32), np.diff(A.xadj))
     perm = np.random.default_rng(7).permutation(rows.size)
     r, c, v = rows[perm], A.cols[perm], A.vals[perm]
-    kind = "reference" if O.ref_mm_available() else "port"
     t0 = time.perf_counter()
-    if kind == "reference":
-        O.ref_coo_sort(A.n, r, c, v)
-    else:
-        O.coo_sort_cel(A.n, r, c, v)
+    O.coo_sort_cel(A.n, r, c, v)
     dt = time.perf_counter() - t0
-    return {"value": rows.size / dt, "unit": "entries/s", "cores": 1, "kind": kind,
+    return {"value": rows.size / dt, "unit": "entries/s", "cores": 1, "kind": "port",
             "seconds": dt, "sample": f"160^3 poisson7 COO, {rows.size} entries, shuffled"}
 
 

@@ -299,16 +299,17 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
         CsrTileStream<double> cs;
         const int t0 = blk.t0, t1 = blk.t1;
         if (t0 < t1) {
-            cs.fetch(S.n, t0, true, S.xadj, S.cols, S.vals);
+            cs.start(S.n, t0, t0 + 1 < t1 ? t0 + 1 : -1, S.xadj, S.cols, S.vals, sm);
             cs.stage(sm);
             __syncthreads();
         }
         for (int t = t0; t < t1; ++t) {
             const int nxt = t + 1 < t1 ? t + 1 : -1;
+            const int aft = t + 2 < t1 ? t + 2 : -1;
             const int row = t * kTileRows + threadIdx.x;
             double pr = 0.0;
-            double sum = cs.template rows<double>(S.n, t, nxt, S.xadj, S.cols, S.vals, ps, sm, row,
-                                                  &pr);
+            double sum = cs.template rows<double>(S.n, t, nxt, aft, S.xadj, S.cols, S.vals, ps, sm,
+                                                  row, &pr);
             __syncthreads();
             if (nxt >= 0) {
                 cs.stage(sm);

@@ -451,8 +451,12 @@ static int spmv_impl(Mat& M, int format, const T* x, T* y, int accumulate, int m
                      const double* ghost, bool distr)
 {
     CEL_TRY(ensure_init());
-    if (format != CEL_FMT_CSR && format != CEL_FMT_JAD && format != CEL_FMT_DIA)
-        return fail(CEL_ERR_ARG, "format must be CSR, JAD or DIA");
+    if (format != CEL_FMT_CSR && format != CEL_FMT_COO && format != CEL_FMT_JAD &&
+        format != CEL_FMT_DIA)
+        return fail(CEL_ERR_ARG, "format must be CSR, COO, JAD or DIA");
+    // the COO product walks the block's row-sorted entries (the CSR arrays)
+    const int coo = format == CEL_FMT_COO;
+    if (coo) format = CEL_FMT_CSR;
     if (format == CEL_FMT_JAD && (M.jad_mode < 0 || M.jad_ndef != M.jad_mmax))
         return fail(CEL_ERR_STATE, "no complete JAD arrays (convert first)");
     if (format == CEL_FMT_DIA && M.dia_nd == 0)
@@ -486,11 +490,13 @@ static int spmv_impl(Mat& M, int format, const T* x, T* y, int accumulate, int m
         }
     }
     if constexpr (f32) {
-        if (format == CEL_FMT_CSR) CEL_TRY(launch_csr_spmv_f32(M, (const float*)dx, (float*)dy, accumulate));
+        if (format == CEL_FMT_CSR)
+            CEL_TRY(launch_csr_spmv_f32(M, (const float*)dx, (float*)dy, accumulate, coo));
         else if (format == CEL_FMT_DIA) CEL_TRY(launch_dia_spmv_f32(M, (const float*)dx, (float*)dy, accumulate));
         else CEL_TRY(launch_jad_spmv_f32(M, (const float*)dx, (float*)dy, accumulate));
     } else {
-        if (format == CEL_FMT_CSR) CEL_TRY(launch_csr_spmv(M, (const double*)dx, (double*)dy, accumulate));
+        if (format == CEL_FMT_CSR)
+            CEL_TRY(launch_csr_spmv(M, (const double*)dx, (double*)dy, accumulate, nullptr, coo));
         else if (format == CEL_FMT_DIA) CEL_TRY(launch_dia_spmv(M, (const double*)dx, (double*)dy, accumulate));
         else CEL_TRY(launch_jad_spmv(M, (const double*)dx, (double*)dy, accumulate));
         if (distr && M.halo_nrows) CEL_TRY(launch_halo_spmv(M, dg, (double*)dy));
@@ -530,13 +536,16 @@ extern "C" int cel_gpu_spmv_time(cel_gpu_mat* A, int format, int precision, cons
     CEL_TRY(ensure_init());
     Mat& M = A->m;
     hipStream_t s = rt().stream;
+    const int coo = format == CEL_FMT_COO;
+    if (coo) format = CEL_FMT_CSR;
     auto once = [&]() -> int {
         if (precision == CEL_PREC_FP64) {
-            if (format == CEL_FMT_CSR) return launch_csr_spmv(M, (const double*)x, (double*)y, 0);
+            if (format == CEL_FMT_CSR)
+                return launch_csr_spmv(M, (const double*)x, (double*)y, 0, nullptr, coo);
             if (format == CEL_FMT_DIA) return launch_dia_spmv(M, (const double*)x, (double*)y, 0);
             return launch_jad_spmv(M, (const double*)x, (double*)y, 0);
         }
-        if (format == CEL_FMT_CSR) return launch_csr_spmv_f32(M, (const float*)x, (float*)y, 0);
+        if (format == CEL_FMT_CSR) return launch_csr_spmv_f32(M, (const float*)x, (float*)y, 0, coo);
         if (format == CEL_FMT_DIA) return launch_dia_spmv_f32(M, (const float*)x, (float*)y, 0);
         return launch_jad_spmv_f32(M, (const float*)x, (float*)y, 0);
     };

@@ -155,6 +155,9 @@ struct Mat {
     DevBuf<int> jad_perm, jad_begin, jad_cols;
     DevBuf<double> jad_vals;
     DevBuf<float> jad_vals32;
+    // the JAD columns renumbered into the permuted row order (pinv[cols]),
+    // for solves that keep their vectors in that order; built on first use
+    DevBuf<int> jad_pcols;
 
     // DIA (cel_gpu_mat_to_dia): values [ndiag][n], presence mask per row
     int dia_nd = 0;                 // 0: no DIA form
@@ -205,9 +208,11 @@ int launch_gen_poisson7(int nx, int ny, int nz, const double* kappa, int* xadj, 
 // skip (may be null): the launch returns at once when *skip is non-zero on
 // the device (for launches captured in a graph whose later replays must do
 // nothing)
+// coo = 1: the COO product of the same (row-sorted) entries, y(row) += a x
+// entry by entry (from y itself when accumulating, else from 0)
 int launch_csr_spmv(const Mat& A, const double* x, double* y, int accumulate,
-                    const int* skip = nullptr);
-int launch_csr_spmv_f32(const Mat& A, const float* x, float* y, int accumulate);
+                    const int* skip = nullptr, int coo = 0);
+int launch_csr_spmv_f32(const Mat& A, const float* x, float* y, int accumulate, int coo = 0);
 int launch_jad_spmv(const Mat& A, const double* x, double* y, int accumulate,
                     const int* skip = nullptr);
 int launch_jad_spmv_f32(const Mat& A, const float* x, float* y, int accumulate);
@@ -240,6 +245,8 @@ int launch_scale_vec(const Mat& A, double* b);
 int launch_to_f32(const double* src, float* dst, size_t n);
 int launch_coo_to_csr(int nrows, int nnz, const int* rows0, int* xadj);  // rows 0-based local
 int jad_convert_device(Mat& A, int mode);
+// Builds A.jad_pcols (once per conversion).
+int jad_prepare_permuted(Mat& A);
 int launch_sddmm(const Mat& A, const double* lambda, const double* x, double* g);
 int launch_kappa_grad(int nx, int ny, int nz, const double* kappa, const double* x,
                       const double* lambda, double* grad);

@@ -263,6 +263,9 @@ extern "C" int cel_gpu_mat_to_dia(cel_gpu_mat* A)
     Mat& M = A->m;
     if (M.dia_nd > 0) return CEL_OK;
     if (M.n == 0 || M.nnz == 0) return fail(CEL_ERR_BAD_MATRIX, "an empty block has no DIA form");
+    // the DIA row sums read x through a resource of n entries (the row
+    // count), so a column index must also be a row index
+    if (M.ncols != M.n) return fail(CEL_ERR_BAD_MATRIX, "a non-square block has no DIA form");
     // the DIA kernels address a diagonal with 32-bit byte offsets
     if ((long long)M.n * (long long)sizeof(double) >= (1LL << 31))
         return fail(CEL_ERR_BAD_MATRIX, "more than 2^28 rows: no DIA form");

@@ -138,6 +138,7 @@ int jad_convert_device(Mat& A, int mode)
     hipStream_t s = rt().stream;
     const int n = A.n;
     A.jad_mode = -1;
+    A.jad_pcols.release();
     if (n == 0) {
         A.jad_mmax = 0;
         A.jad_ilo = 0;
@@ -244,6 +245,42 @@ int jad_convert_device(Mat& A, int mode)
         return fail(CEL_ERR_REF_UNDEFINED,
                     "CEL_COMPAT JAD: shortest row is non-empty, the reference's diagonal counts "
                     "beyond the defined ones are uninitialised");
+    return CEL_OK;
+}
+
+// pinv[perm[k]] = k
+__global__ void k_inv_perm(int n, const int* __restrict__ perm, int* __restrict__ pinv)
+{
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x)
+        pinv[perm[k]] = (int)k;
+}
+
+// out[e] = pinv[cols[e]]
+__global__ void k_remap_cols(long long nnz, const int* __restrict__ cols,
+                             const int* __restrict__ pinv, int* __restrict__ out)
+{
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
+         e += (long long)gridDim.x * blockDim.x)
+        out[e] = pinv[cols[e]];
+}
+
+int jad_prepare_permuted(Mat& A)
+{
+    if (A.jad_pcols.p || A.jad_nnz == 0) return CEL_OK;
+    if (A.ncols != A.n) return fail(CEL_ERR_ARG, "the permuted JAD form needs a square block");
+    hipStream_t s = rt().stream;
+    DevBuf<int> pinv;
+    CEL_TRY(pinv.alloc(A.n));
+    CEL_TRY(A.jad_pcols.alloc(A.jad_nnz));
+    const int g = std::max(1, std::min((A.n + kJadBlock - 1) / kJadBlock, rt().num_cus * 8));
+    hipLaunchKernelGGL(k_inv_perm, dim3(g), dim3(kJadBlock), 0, s, A.n, A.jad_perm.p, pinv.p);
+    const long long ge = std::max<long long>(
+        1, std::min<long long>((A.jad_nnz + kJadBlock - 1) / kJadBlock, rt().num_cus * 8));
+    hipLaunchKernelGGL(k_remap_cols, dim3((int)ge), dim3(kJadBlock), 0, s, A.jad_nnz, A.jad_cols.p,
+                       pinv.p, A.jad_pcols.p);
+    CEL_HIP(hipGetLastError());
+    CEL_HIP(hipStreamSynchronize(s));  // pinv is released on return
     return CEL_OK;
 }
 

@@ -136,8 +136,29 @@ __global__ void k_fill(long long n, double* __restrict__ x, double v)
         x[i] = v;
 }
 
+// dst[k] = src[perm[k]] (to the permuted row order) and back
+template <typename T>
+__global__ void k_perm_gather(long long n, const int* __restrict__ perm, const T* __restrict__ src,
+                              T* __restrict__ dst)
+{
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x)
+        dst[k] = src[perm[k]];
+}
+
+template <typename T>
+__global__ void k_perm_scatter(long long n, const int* __restrict__ perm, const T* __restrict__ src,
+                               T* __restrict__ dst)
+{
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x)
+        dst[perm[k]] = src[k];
+}
+
 struct MatView {
     int n, mmax;
+    int coo;  // the local block is applied as COO: v(row) += a p entry by entry from v = 0
+    long long jnnz;
     const int *xadj, *cols;
     const double* vals;
     const float* vals32;
@@ -150,6 +171,7 @@ struct MatView {
     const uint32_t* dmask;   // presence bits, more than 8 diagonals
     const uint8_t* dmask8;   // presence bits, at most 8 diagonals
     int nd;
+    int dzero;               // index of the diagonal with offset 0, or -1
     int hbit;                // with halo: mask bit of the rows that have halo entries
     int doff[32];
     // halo (FP64 only)
@@ -195,7 +217,8 @@ struct TileWalk {
     {
         if constexpr (FMT == CEL_FMT_CSR) {
             if (t0 < t1) {  // uniform across the block
-                cs.fetch(A.n, (int)t0, true, A.xadj, A.cols, csr_vals<VT>(A));
+                cs.start(A.n, (int)t0, t0 + 1 < t1 ? (int)t0 + 1 : -1, A.xadj, A.cols,
+                         csr_vals<VT>(A), sm);
                 cs.stage(sm);
                 __syncthreads();
             }
@@ -210,28 +233,35 @@ struct TileWalk {
         AT s;
         if constexpr (FMT == CEL_FMT_CSR) {
             const int nxt = (t + 1 < t1) ? (int)(t + 1) : -1;
+            const int aft = (t + 2 < t1) ? (int)(t + 2) : -1;
             row = r0 + threadIdx.x;
-            s = cs.template rows<AT>(A.n, (int)t, nxt, A.xadj, A.cols, csr_vals<VT>(A), x, sm, row,
-                                     xrow);
+            s = cs.template rows<AT>(A.n, (int)t, nxt, aft, A.xadj, A.cols, csr_vals<VT>(A), x, sm,
+                                     row, xrow);
             __syncthreads();  // every row of the tile is summed
             if (nxt >= 0) {   // uniform across the block
                 cs.stage(sm);
                 __syncthreads();
             }
-            s = AT(0) + s;
+            // CSR: r = 0; r = r + tmp.  COO: the entries were added to 0 one
+            // by one (the same bits but for the sign of a zero sum)
+            s = A.coo ? s : AT(0) + s;
             if constexpr (HALO) {
                 if (row < A.n)
                     s = halo_row<AT>(row, (int)t, s, A.hrow_tile, A.hrows, A.hptr, A.hcols,
                                      A.hvals, A.ghost);
             }
         } else {
+            // JAD solves keep their vectors in the permuted row order, with
+            // the columns renumbered into it: row k of the permuted system
             const int k = r0 + threadIdx.x;
             s = AT(0);
             row = A.n;
             if (k < A.n) {
+                VT xr = VT(0);
+                if (xrow) xr = x[k];
                 s = jad_row<VT, AT>(k, A.mmax, A.jbegin, A.jcols, jad_vals<VT>(A), x);
-                row = A.jperm[k];
-                if (xrow) *xrow = x[row];
+                row = k;
+                if (xrow) *xrow = xr;
             }
         }
         return s;
@@ -313,9 +343,11 @@ __global__ __launch_bounds__(kBlock) void k_residual(MatView A, const double* __
 }
 
 // The CSR, DIA and JAD K1s walk their tiles in group_tile's order
-// (spmv_device.hpp); all forms add a thread's rows to the block partial in
-// that order, so their partials and solves are bit-identical.  (Groups of
-// 4 tiles measured: FP64 DIA solve 1% faster, mixed DIA 1% slower.)
+// (spmv_device.hpp).  The CSR and DIA K1s add a thread's rows to the block
+// partial in that order, so their partials and solves are bit-identical;
+// the JAD K1's tiles hold permuted rows, so it shares only the order of the
+// tiles, and its solves agree with the others to rounding.  (Groups of 4
+// tiles measured: FP64 DIA solve 1% faster, mixed DIA 1% slower.)
 
 // K1: v = A p, per-block partial of v.p.  Folds pending set-up or
 // replacement partials into the token first (every block, same order) and
@@ -380,7 +412,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
         }
         const long long tfirst = group_tile(0, ntiles);
         if (tfirst >= 0) {  // uniform across the block
-            cs.fetch(A.n, (int)tfirst, true, A.xadj, A.cols, vals);
+            cs.start(A.n, (int)tfirst, (int)group_tile(1, ntiles), A.xadj, A.cols, vals, sm);
             cs.stage(sm);
             if constexpr (HALO)
                 hstaged = halo_stage_mark(hs, (int)tfirst, A.hrow_tile, A.hrows, A.hptr, A.hcols,
@@ -390,10 +422,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
         long long k = 0;
         for (long long t = tfirst; t >= 0; t = group_tile(++k, ntiles)) {
             const int nxt = (int)group_tile(k + 1, ntiles);
+            const int aft = (int)group_tile(k + 2, ntiles);
             const int row = (int)t * kTileRows + threadIdx.x;
             VT pr = VT(0);
-            VT s = cs.template rows<VT>(A.n, (int)t, nxt, A.xadj, A.cols, vals, p, sm, row, &pr);
-            s = VT(0) + s;
+            VT s = cs.template rows<VT>(A.n, (int)t, nxt, aft, A.xadj, A.cols, vals, p, sm, row,
+                                        &pr);
+            s = A.coo ? s : VT(0) + s;
             if constexpr (HALO)
                 s = halo_stage_add<VT>(hs, (int)t, hstaged, s, A.hcols, A.hvals, A.ghost);
             __syncthreads();  // the tile's LDS stage and halo products are consumed
@@ -423,13 +457,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
                 r[k] = t < ntiles ? (int)t * kTileRows + threadIdx.x : A.n;
             }
             if (r[0] < A.n) {
-                VT s[R];
+                VT s[R], pr[R];
+                // p[row] comes from the loads of the diagonal with offset 0
                 dia_rows<VT, R, (R >= 4 ? 4 : 8)>(A.n, A.nd, A.doff, dia_vals<VT>(A), A.dmask8,
-                                                   A.dmask, r, p, s);
+                                                   A.dmask, r, p, s, pr, A.dzero);
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
                     if (r[k] < A.n) {
-                        const VT pk = p[r[k]];
+                        const VT pk = A.dzero >= 0 ? pr[k] : p[r[k]];
                         VT sk = VT(0) + s[k];
                         if constexpr (HALO) {
                             // rows flagged in the masks add their halo
@@ -452,15 +487,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
             }
         }
     } else {
-        // JAD: the CSR K1's tile-pair order (one permuted row per thread
-        // and tile; the tile walk needs no staging)
-        TileWalk<VT, VT, FMT, false> walk;
+        // JAD in the permuted row order: the CSR K1's tile-pair order, one
+        // row per thread and tile, the next tile's row loaded while this
+        // one's gathers are in flight
+        JadStream<VT> js;
+        js.init(A.n, A.mmax, A.jnnz, A.jbegin, A.jcols, jad_vals<VT>(A), p);
         long long k = 0;
-        for (long long t = group_tile(0, ntiles); t >= 0; t = group_tile(++k, ntiles)) {
-            int row;
-            VT pr = VT(0);
-            VT s = walk.next(A, t, t + 1, p, sm, row, &pr);
+        long long t = group_tile(0, ntiles);
+        if (t >= 0) js.fetch((int)t * kTileRows + threadIdx.x);
+        for (; t >= 0; t = group_tile(++k, ntiles)) {
+            const long long tn = group_tile(k + 1, ntiles);
+            const int row = (int)t * kTileRows + threadIdx.x;
+            const int rn = tn >= 0 ? (int)tn * kTileRows + threadIdx.x : A.n;
+            VT pr;
+            VT s = js.template row<VT>(row, rn, A.mmax, A.jbegin, A.jcols, jad_vals<VT>(A), p, pr);
             if (row < A.n) {
+                // the reference's JAD product overwrites v (y(perm(k)) = tmp(k))
                 v[row] = s;
                 double prod = double(s) * double(pr);
                 acc = acc + prod;
@@ -711,6 +753,9 @@ __global__ __launch_bounds__(kBlock) void k_ru_flush(long long n, double* __rest
 struct Workspace {
     DevBuf<double> r, p, v, x, b, part1, part2, rpart, ghost, sendbuf, hist;
     DevBuf<float> r32, p32, v32, e32;
+    // JAD solves: b, x and the Jacobi vector in the permuted row order
+    DevBuf<double> bp, xp, jacp;
+    DevBuf<float> jacp32;
     DevBuf<Ctrl> ctrl;
     DevBuf<int> hrow_tile;
 };
@@ -748,6 +793,14 @@ static int build_hrow_tile(const Mat& A, Workspace& W, int ntiles)
 // partials, so more blocks cost more reduction work.
 constexpr int kVecBlocksPerCu = 4;
 
+// Blocks per CU of the K1s (at most the resident blocks).  Measured on the
+// 256^3 block (mean K1 time over 100-iteration solves, three interleaved
+// repetitions on one box): FP32 4 per CU (CSR 206-223 us, DIA 121-127 us,
+// against 223-230 / 136-145 us with every resident block), FP64 2 per CU
+// (CSR 349-352 against 376 us, DIA 260-264 against 266-271 us).
+constexpr int kK1BlocksPerCuF32 = 4;
+constexpr int kK1BlocksPerCuF64 = 2;
+
 static int common_grid(std::initializer_list<const void*> kernels, long long work)
 {
     int g = 1 << 30;
@@ -762,10 +815,13 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     hipStream_t s = rt().stream;
     const int n = A.n;
     const bool mixed = prm.precision == CEL_PREC_MIXED;
-    const int fmt = prm.format;
+    // COO (cel_fop_mv_module.f90:357-403) walks the block's row-sorted
+    // entries with the CSR kernels, adding each product to v in entry order
+    const bool coo = prm.format == CEL_FMT_COO;
+    const int fmt = coo ? CEL_FMT_CSR : prm.format;
     const bool halo = prm.use_halo != 0;
     if (fmt != CEL_FMT_CSR && fmt != CEL_FMT_JAD && fmt != CEL_FMT_DIA)
-        return fail(CEL_ERR_ARG, "format must be CSR, JAD or DIA");
+        return fail(CEL_ERR_ARG, "format must be CSR, COO, JAD or DIA");
     if (fmt == CEL_FMT_JAD && (A.jad_mode < 0 || A.jad_ndef != A.jad_mmax))
         return fail(CEL_ERR_STATE, "JAD requested but the matrix has no complete JAD arrays");
     if (fmt == CEL_FMT_DIA && A.dia_nd == 0)
@@ -779,6 +835,17 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         stats = cel_pcg_stats{};
         stats.status = CEL_OK;
         return CEL_OK;
+    }
+
+    // a JAD solve runs in the permuted row order (vectors gathered on entry,
+    // x scattered back on exit), with the columns renumbered into it
+    const bool jadp = fmt == CEL_FMT_JAD;
+    if (jadp) {
+        // the JAD K1 addresses the entries with unsigned 32-bit byte offsets
+        const double lim = 4294967296.0 - 64.0;
+        if ((double)A.jad_nnz * (mixed ? 4.0 : 8.0) >= lim || (double)n * 8.0 >= lim)
+            return fail(CEL_ERR_ARG, "JAD solve: entries beyond the 32-bit offsets of the JAD kernel");
+        CEL_TRY(jad_prepare_permuted(A));
     }
 
     Workspace& W = ws_of(A);
@@ -814,6 +881,10 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         }
     }
     {
+        const int cap = rt().num_cus * (mixed ? kK1BlocksPerCuF32 : kK1BlocksPerCuF64);
+        if (G1 > cap) G1 = cap >= kXcds ? cap / kXcds * kXcds : cap;
+    }
+    {
         const long long units = ((long long)n + 2 * kBlock - 1) / (2 * kBlock);
         Gv = (int)std::min<long long>((long long)rt().num_cus * kVecBlocksPerCu,
                                       std::max<long long>(1, units));
@@ -823,7 +894,9 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     // matrix data for the mixed inner operator (once per matrix)
     if (mixed) {
         if (fmt == CEL_FMT_DIA) CEL_TRY(dia_prepare_f32(A));
-        if ((fmt == CEL_FMT_CSR || fmt == CEL_FMT_DIA) && !A.vals32.p) {
+        // the DIA K1 reads dia_vals32 and the residual kernels the FP64 CSR
+        // values, so only a CSR solve needs the FP32 CSR copy
+        if (fmt == CEL_FMT_CSR && !A.vals32.p) {
             CEL_TRY(A.vals32.alloc(A.nnz > 0 ? A.nnz : 1));
             CEL_TRY(launch_to_f32(A.vals.p, A.vals32.p, A.nnz));
         }
@@ -860,6 +933,17 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         db = W.b.p;
         dx = W.x.p;
     }
+    double* const dx_out = dx;
+    const double* jac64 = A.jac.p;
+    const float* jac32 = A.jac32.p;
+    if (jadp) {
+        CEL_TRY(W.bp.alloc(nn));
+        CEL_TRY(W.xp.alloc(nn));
+        CEL_TRY(W.jacp.alloc(nn));
+        if (mixed) CEL_TRY(W.jacp32.alloc(nn));
+        jac64 = W.jacp.p;
+        jac32 = mixed ? W.jacp32.p : nullptr;
+    }
     const int hist_cap = hist ? prm.iter_max + 1 : 0;
     if (hist_cap) CEL_TRY(W.hist.alloc(hist_cap));
     if (halo) {
@@ -894,13 +978,15 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
 
     MatView V{};
     V.n = n;
+    V.coo = coo ? 1 : 0;
     V.mmax = A.jad_mmax;
+    V.jnnz = A.jad_nnz;
     V.xadj = A.xadj.p;
     V.cols = A.cols.p;
     V.vals = A.vals.p;
     V.vals32 = A.vals32.p;
     V.jbegin = A.jad_begin.p;
-    V.jcols = A.jad_cols.p;
+    V.jcols = jadp ? A.jad_pcols.p : A.jad_cols.p;
     V.jperm = A.jad_perm.p;
     V.jvals = A.jad_vals.p;
     V.jvals32 = A.jad_vals32.p;
@@ -910,7 +996,11 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     V.dmask8 = A.dia_mask8.p;
     V.nd = A.dia_nd;
     V.hbit = hbit;
-    for (int d = 0; d < A.dia_nd; ++d) V.doff[d] = A.dia_off[d];
+    V.dzero = -1;
+    for (int d = 0; d < A.dia_nd; ++d) {
+        V.doff[d] = A.dia_off[d];
+        if (A.dia_off[d] == 0) V.dzero = d;
+    }
     if (halo) {
         V.hrow_tile = W.hrow_tile.p;
         V.hrows = A.halo_rows.p;
@@ -927,6 +1017,22 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     CEL_TRY(R.event(e1));
     CEL_TRY(R.event(e2));
     CEL_HIP(hipEventRecord(e0, s));
+    if (jadp) {
+        const int gp = std::max(1, std::min(Gv * 2, 4096));
+        hipLaunchKernelGGL(k_perm_gather<double>, dim3(gp), dim3(kBlock), 0, s, (long long)n,
+                           A.jad_perm.p, db, W.bp.p);
+        hipLaunchKernelGGL(k_perm_gather<double>, dim3(gp), dim3(kBlock), 0, s, (long long)n,
+                           A.jad_perm.p, A.jac.p, W.jacp.p);
+        if (mixed)
+            hipLaunchKernelGGL(k_perm_gather<float>, dim3(gp), dim3(kBlock), 0, s, (long long)n,
+                               A.jad_perm.p, A.jac32.p, W.jacp32.p);
+        if (prm.x0_mode == CEL_X0_USER)
+            hipLaunchKernelGGL(k_perm_gather<double>, dim3(gp), dim3(kBlock), 0, s, (long long)n,
+                               A.jad_perm.p, dx, W.xp.p);
+        CEL_HIP(hipGetLastError());
+        db = W.bp.p;
+        dx = W.xp.p;
+    }
 
     // x0 (cel_cgalg_module.f90:209 uses 1.0)
     if (prm.x0_mode != CEL_X0_USER)
@@ -937,7 +1043,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     // r0, p0 and their partials; the first K1 folds them in.  The set-up
     // reads and writes token slot 0 (only block 0 writes, after its read).
 #define RES_ARGS(OT, R, P_)                                                                \
-    V, dx, db, A.jac.p, A.jac32.p, (OT*)(R), (OT*)(P_), W.e32.p, W.rpart.p, C0, C0, P
+    V, dx, db, jac64, jac32, (OT*)(R), (OT*)(P_), W.e32.p, W.rpart.p, C0, C0, P
     if (!mixed) {
         if (rfmt == CEL_FMT_CSR && halo)
             hipLaunchKernelGGL((k_residual<double, CEL_FMT_CSR, true, false>), dim3(Gr), dim3(kBlock), 0, s,
@@ -1000,9 +1106,9 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
                                    W.p.p, W.v.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
             if (profile) CEL_HIP(hipEventRecord(eb, s));
             hipLaunchKernelGGL((k_pcg_update_r<double>), dim3(Gv), dim3(kBlock), 0, s, n_ll, W.r.p, W.v.p,
-                               A.jac.p, W.part2.p, W.part1.p, G1, C1, C2);
+                               jac64, W.part2.p, W.part1.p, G1, C1, C2);
             hipLaunchKernelGGL((k_pcg_update_xp<double, double, false>), dim3(Gv), dim3(kBlock), 0, s, n_ll,
-                               dx, W.p.p, W.r.p, A.jac.p, nullptr, W.part2.p, Gv, C2, C0, P);
+                               dx, W.p.p, W.r.p, jac64, nullptr, W.part2.p, Gv, C2, C0, P);
         } else {
             if (fmt == CEL_FMT_CSR)
                 hipLaunchKernelGGL((k_pcg_spmv<float, CEL_FMT_CSR, false>), dim3(G1), dim3(kBlock), 0, s, V,
@@ -1015,16 +1121,16 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
                                    W.p32.p, W.v32.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
             if (profile) CEL_HIP(hipEventRecord(eb, s));
             hipLaunchKernelGGL((k_pcg_update_r<float>), dim3(Gv), dim3(kBlock), 0, s, n_ll, W.r32.p, W.v32.p,
-                               A.jac32.p, W.part2.p, W.part1.p, G1, C1, C2);
+                               jac32, W.part2.p, W.part1.p, G1, C1, C2);
             hipLaunchKernelGGL((k_pcg_update_xp<float, float, true>), dim3(Gv), dim3(kBlock), 0, s, n_ll,
-                               W.e32.p, W.p32.p, W.r32.p, A.jac32.p, dx, W.part2.p, Gv, C2, C3, P);
+                               W.e32.p, W.p32.p, W.r32.p, jac32, dx, W.part2.p, Gv, C2, C3, P);
             if (rfmt == CEL_FMT_CSR)
                 hipLaunchKernelGGL((k_residual<float, CEL_FMT_CSR, false, true>), dim3(Gr), dim3(kBlock), 0, s,
-                                   V, dx, db, A.jac.p, A.jac32.p, W.r32.p, W.p32.p, W.e32.p, W.rpart.p, C3,
+                                   V, dx, db, jac64, jac32, W.r32.p, W.p32.p, W.e32.p, W.rpart.p, C3,
                                    C0, P);
             else
                 hipLaunchKernelGGL((k_residual<float, CEL_FMT_JAD, false, true>), dim3(Gr), dim3(kBlock), 0, s,
-                                   V, dx, db, A.jac.p, A.jac32.p, W.r32.p, W.p32.p, W.e32.p, W.rpart.p, C3,
+                                   V, dx, db, jac64, jac32, W.r32.p, W.p32.p, W.e32.p, W.rpart.p, C3,
                                    C0, P);
         }
         CEL_HIP(hipGetLastError());
@@ -1083,6 +1189,11 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         // a budget that ran out mid-way leaves a correction in e: fold it in
         hipLaunchKernelGGL(k_ru_flush, dim3(Gv), dim3(kBlock), 0, s, n_ll, dx, W.e32.p, C0, C3, 1);
     }
+    if (jadp) {  // x back to the original row order
+        hipLaunchKernelGGL(k_perm_scatter<double>, dim3(std::max(1, std::min(Gv * 2, 4096))),
+                           dim3(kBlock), 0, s, n_ll, A.jad_perm.p, dx, dx_out);
+        CEL_HIP(hipGetLastError());
+    }
     CEL_HIP(hipEventRecord(e2, s));
     CEL_HIP(hipStreamSynchronize(s));
     if (status < 0) return status;
@@ -1103,7 +1214,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     }
 
     if (prm.memspace == CEL_MEM_HOST)
-        CEL_HIP(hipMemcpy(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost));
+        CEL_HIP(hipMemcpy(x, dx_out, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (hist && hist_cap) {
         int cnt = std::min(out.iter + 1, hist_cap);
         CEL_HIP(hipMemcpy(hist, W.hist.p, sizeof(double) * cnt, hipMemcpyDeviceToHost));
@@ -1121,6 +1232,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     stats.spmv_launches = spmv_n;
     const int prec = mixed ? CEL_PREC_MIXED : CEL_PREC_FP64;
     stats.spmv_bytes = spmv_bytes(A, fmt, prec);
+    if (jadp) stats.spmv_bytes -= 4.0 * n;  // the permuted-order K1 reads no permutation
     {
         const double vb = mixed ? 4.0 : 8.0;
         // K1 matrix + p gather + v write ; K2 r,v,jac read + r write ;

@@ -4,13 +4,20 @@
 
 namespace cel {
 
-// ------------------------------------------------------------------ CSR / JAD SpMV
-template <typename VT>
+// ------------------------------------------------------------------ CSR / COO / JAD SpMV
+// CSR (cel_fop_mv_module.f90:215-229): y(i) = y(i) + tmp, tmp the row sum
+// from 0 (y zeroed first unless accumulating).  COO
+// (cel_fop_mv_module.f90:380-391): y(row) = y(row) + a x for each entry in
+// order; the block's entries are row-sorted (the order its CSR offsets
+// assume, cel_sp_mat_module.f90:430-456), so a row's run of entries is
+// added to y one by one, in entry order, with no atomics: the row sum
+// starts from y(row) itself (INIT) or from 0 when y is overwritten.
+template <typename VT, bool INIT>
 __global__ __launch_bounds__(kBlock) void k_csr_spmv(int n, const int* __restrict__ xadj,
                                                      const int* __restrict__ cols,
                                                      const VT* __restrict__ vals,
                                                      const VT* __restrict__ x, VT* __restrict__ y,
-                                                     int accumulate, int ntiles,
+                                                     int accumulate, int coo, int ntiles,
                                                      const int* __restrict__ skip)
 {
     if (skip && *skip) return;  // uniform across the grid
@@ -18,7 +25,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmv(int n, const int* __restric
     CsrTileStream<VT> ts;
     const int tfirst = (int)group_tile(0, ntiles);
     if (tfirst >= 0) {  // uniform across the block
-        ts.fetch(n, tfirst, true, xadj, cols, vals);
+        ts.start(n, tfirst, (int)group_tile(1, ntiles), xadj, cols, vals, sm);
         ts.stage(sm);
         __syncthreads();
     }
@@ -26,13 +33,16 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmv(int n, const int* __restric
     for (int t = tfirst; t >= 0; t = (int)group_tile(++step, ntiles)) {
         const int row = t * kTileRows + threadIdx.x;
         const int nxt = (int)group_tile(step + 1, ntiles);
-        VT s = ts.template rows<VT>(n, t, nxt, xadj, cols, vals, x, sm, row, nullptr);
+        const int aft = (int)group_tile(step + 2, ntiles);
+        VT init = VT(0);
+        if constexpr (INIT) init = y[row < n ? row : 0];
+        VT s = ts.template rows<VT>(n, t, nxt, aft, xadj, cols, vals, x, sm, row, nullptr, init);
         __syncthreads();
         if (nxt >= 0) {
             ts.stage(sm);
             __syncthreads();
         }
-        if (row < n) y[row] = accumulate ? y[row] + s : VT(0) + s;
+        if (row < n) y[row] = coo ? s : accumulate ? y[row] + s : VT(0) + s;
     }
 }
 
@@ -60,7 +70,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmv_fused(int n, long long chun
     const int t1 = c0 < c1 ? (int)((c1 + kTileRows - 1) / kTileRows) : t0;
     CsrTileStream<double> ts;
     if (t0 < t1) {  // uniform across the block
-        ts.fetch(n, t0, true, xadj, cols, vals);
+        ts.start(n, t0, t0 + 1 < t1 ? t0 + 1 : -1, xadj, cols, vals, sm);
         ts.stage(sm);
         __syncthreads();
     }
@@ -68,7 +78,8 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmv_fused(int n, long long chun
     for (int t = t0; t < t1; ++t) {
         const int row = t * kTileRows + threadIdx.x;
         const int nxt = t + 1 < t1 ? t + 1 : -1;
-        double s = ts.template rows<double>(n, t, nxt, xadj, cols, vals, x, sm, row, nullptr);
+        const int aft = t + 2 < t1 ? t + 2 : -1;
+        double s = ts.template rows<double>(n, t, nxt, aft, xadj, cols, vals, x, sm, row, nullptr);
         __syncthreads();
         if (nxt >= 0) {
             ts.stage(sm);
@@ -113,16 +124,30 @@ static int spmv_no_columns(const Mat& A, T* y, int accumulate)
     return CEL_OK;
 }
 
-int launch_csr_spmv(const Mat& A, const double* x, double* y, int accumulate, const int* skip)
+template <typename VT>
+static int launch_csr_tiles(const Mat& A, const VT* vals, const VT* x, VT* y, int accumulate,
+                            int coo, const int* skip)
+{
+    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
+    if (coo && accumulate) {
+        const int G = resident_grid((const void*)k_csr_spmv<VT, true>, ntiles);
+        hipLaunchKernelGGL((k_csr_spmv<VT, true>), dim3(G), dim3(kBlock), 0, rt().stream, A.n,
+                           A.xadj.p, A.cols.p, vals, x, y, accumulate, coo, ntiles, skip);
+    } else {
+        const int G = resident_grid((const void*)k_csr_spmv<VT, false>, ntiles);
+        hipLaunchKernelGGL((k_csr_spmv<VT, false>), dim3(G), dim3(kBlock), 0, rt().stream, A.n,
+                           A.xadj.p, A.cols.p, vals, x, y, accumulate, coo, ntiles, skip);
+    }
+    CEL_HIP(hipGetLastError());
+    return CEL_OK;
+}
+
+int launch_csr_spmv(const Mat& A, const double* x, double* y, int accumulate, const int* skip,
+                    int coo)
 {
     if (A.n == 0) return CEL_OK;
     if (A.ncols == 0) return spmv_no_columns(A, y, accumulate);
-    int ntiles = (A.n + kTileRows - 1) / kTileRows;
-    int G = resident_grid((const void*)k_csr_spmv<double>, ntiles);
-    hipLaunchKernelGGL(k_csr_spmv<double>, dim3(G), dim3(kBlock), 0, rt().stream, A.n, A.xadj.p,
-                       A.cols.p, A.vals.p, x, y, accumulate, ntiles, skip);
-    CEL_HIP(hipGetLastError());
-    return CEL_OK;
+    return launch_csr_tiles<double>(A, A.vals.p, x, y, accumulate, coo, skip);
 }
 
 int launch_csr_spmv_fused(const Mat& A, int fuse, long long chunk, int G, const double* x,
@@ -144,18 +169,12 @@ int launch_csr_spmv_fused(const Mat& A, int fuse, long long chunk, int G, const 
     return CEL_OK;
 }
 
-int launch_csr_spmv_f32(const Mat& A, const float* x, float* y, int accumulate)
+int launch_csr_spmv_f32(const Mat& A, const float* x, float* y, int accumulate, int coo)
 {
     if (A.n == 0) return CEL_OK;
     if (A.ncols == 0) return spmv_no_columns(A, y, accumulate);
     if (!A.vals32.p) return fail(CEL_ERR_STATE, "FP32 values not prepared");
-    int ntiles = (A.n + kTileRows - 1) / kTileRows;
-    int G = resident_grid((const void*)k_csr_spmv<float>, ntiles);
-    hipLaunchKernelGGL(k_csr_spmv<float>, dim3(G), dim3(kBlock), 0, rt().stream, A.n, A.xadj.p,
-                       A.cols.p, A.vals32.p, x, y, accumulate, ntiles,
-                       nullptr);
-    CEL_HIP(hipGetLastError());
-    return CEL_OK;
+    return launch_csr_tiles<float>(A, A.vals32.p, x, y, accumulate, coo, nullptr);
 }
 
 int launch_jad_spmv(const Mat& A, const double* x, double* y, int accumulate, const int* skip)

@@ -146,11 +146,15 @@ __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int v
 // resource per diagonal), x through one buffer resource, so the x of an
 // absent boundary entry reads 0 from outside the resource instead of
 // needing a clamp.  n * sizeof(VT) must stay below 2^31 (32-bit byte
-// offsets).
+// offsets).  With xr, x[r[k]] is returned in xr[k] from the loads of the
+// diagonal with offset 0 (index dzero >= 0, present in the caller's
+// layout), taken after the products' wait, so it costs no load and no
+// round trip of its own.
 template <typename VT, int R, int GD>
 __device__ __forceinline__ void dia_rows(int n, int nd, const int* doff, const VT* vals,
                                          const uint8_t* m8, const uint32_t* m32, const int (&r)[R],
-                                         const VT* __restrict__ x, VT (&s)[R])
+                                         const VT* __restrict__ x, VT (&s)[R], VT* xr = nullptr,
+                                         int dzero = -1)
 {
     constexpr int B = (int)sizeof(VT);
     int rc[R];
@@ -189,17 +193,31 @@ __device__ __forceinline__ void dia_rows(int n, int nd, const int* doff, const V
                     const VT prod = v[g][k] * xv[g][k];
                     s[k] = ((m[k] >> d) & 1u) ? s[k] + prod : s[k];
                 }
+                if (xr && d == dzero) {
+#pragma unroll
+                    for (int k = 0; k < R; ++k) xr[k] = xv[g][k];
+                }
             }
         }
     }
 }
 
-// Software-pipelined CSR tile stream.  fetch(t) issues the loads of tile t
-// (row bounds of every thread and, when the tile fits the stage, its
-// entries into registers); stage() parks them in LDS; rows() sums the
-// calling thread's row of the staged tile while the next tile loads:
-//   fetch(t0); stage; sync
-//   for t: s = rows(t, next = t+1); sync; stage; sync
+// Software-pipelined CSR tile stream.  bounds(t) loads the entry range of
+// tile t (two row pointers); fetch(t) issues the loads of tile t (its
+// entries into registers when the tile fits the stage, and the calling
+// thread's row bounds) from the range bounds() loaded one step earlier;
+// stage() parks the entries in LDS; rows() sums the calling thread's row of
+// the staged tile while the next tile loads:
+//   bounds(t0); fetch(t0); bounds(t1); stage; sync
+//   for t: s = rows(t, next = t1, after = t2); sync; stage; sync
+// rows() issues, in this order: the gathers of x for tile t, the loads of
+// tile t1, the row bounds of t1 and the entry range of t2.  Vector memory
+// operations complete in issue order, so nothing in a step waits for a load
+// issued later in it: the sum waits for the gathers only, the stage for the
+// tile's entries only.  (With the entry range of t1 loaded in the same step
+// as its entries, the tile's loads had to wait for the gathers issued before
+// them, which put one L2 round trip and one HBM round trip in series on
+// every step.)
 template <typename VT>
 struct CsrTileStream {
     static constexpr int EV = 16 / (int)sizeof(VT);  // values per 16-byte load
@@ -208,12 +226,23 @@ struct CsrTileStream {
     int e0, cnt;  // first entry and entries of the fetched tile (uniform)
     int lo, hi;   // the calling thread's row bounds (lo = hi = e0 past n)
     int sv, sc;   // entries the 16-byte value / column loads start before e0
+    int pb0, pb1; // entry range of the tile bounds() was called for
 
-    // Loads of tile t.  Unconditional by design: with want = false (no next
-    // tile; t is then the current tile) or a tile larger than the stage,
-    // the buffer covers 0 bytes and the loads return 0 without touching
-    // memory, so the registers never merge loaded and stale values (such
-    // merges force a wait for the loads before the current tile's work).
+    // Entry range of tile t (t < 0: tile 0, a valid load whose value the
+    // next fetch uses with want = false).
+    __device__ __forceinline__ void bounds(int n, int t, const int* __restrict__ xadj)
+    {
+        const int r0 = (t >= 0 ? t : 0) * kTileRows;
+        pb0 = xadj[r0];
+        pb1 = xadj[r0 + min(kTileRows, n - r0)];
+    }
+
+    // Loads of tile t, whose entry range the last bounds() call loaded.
+    // Unconditional by design: with want = false (no next tile) or a tile
+    // larger than the stage, the buffer covers 0 bytes and the loads return
+    // 0 without touching memory, so the registers never merge loaded and
+    // stale values (such merges force a wait for the loads before the
+    // current tile's work).
     __device__ __forceinline__ void fetch(int n, int t, bool want, const int* __restrict__ xadj,
                                           const int* __restrict__ cols,
                                           const VT* __restrict__ vals)
@@ -221,11 +250,10 @@ struct CsrTileStream {
         const int tid = threadIdx.x;
         const int r0 = t * kTileRows;
         const int nrows = min(kTileRows, n - r0);
-        e0 = xadj[r0];
-        cnt = xadj[r0 + nrows] - e0;
-        // threads past the last row get lo = hi (an empty row)
-        lo = xadj[r0 + min(tid, nrows)];
-        hi = xadj[r0 + min(tid + 1, nrows)];
+        // every lane loaded the same two words; take them as scalars so the
+        // buffer resources are built in scalar registers
+        e0 = __builtin_amdgcn_readfirstlane(pb0);
+        cnt = __builtin_amdgcn_readfirstlane(pb1) - e0;
         const int m = (want && cnt <= kStageMax) ? cnt : 0;
         // 16-byte loads from the 16-byte boundary at or before e0: thread
         // tid holds the EV (values) or 4 (columns) consecutive entries at
@@ -255,6 +283,25 @@ struct CsrTileStream {
 #pragma unroll
             for (int j = 0; j < 4; ++j) c[i * 4 + j] = (int)w[j];
         }
+        // threads past the last row get lo = hi (an empty row)
+        lo = xadj[r0 + min(tid, nrows)];
+        hi = xadj[r0 + min(tid + 1, nrows)];
+    }
+
+    // First tile of a walk: clears the staged columns (rows() gathers
+    // through stale slots, which must hold valid columns), then its range,
+    // its loads and the range of the tile after it.  Every thread of the
+    // block calls it (it synchronises); the caller stages and synchronises.
+    __device__ __forceinline__ void start(int n, int t, int next, const int* __restrict__ xadj,
+                                          const int* __restrict__ cols,
+                                          const VT* __restrict__ vals, CsrStage<VT>& sm)
+    {
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) sm.col[threadIdx.x + kBlock * i] = 0;
+        __syncthreads();
+        bounds(n, t, xadj);
+        fetch(n, t, true, xadj, cols, vals);
+        bounds(n, next, xadj);
     }
 
     // Parks the fetched entries in LDS (tiles that fit the stage).
@@ -278,59 +325,64 @@ struct CsrTileStream {
         }
     }
 
-    // Row sum of the staged tile t; starts fetching tile `next` (< 0: none).
-    // The gathers of x (and x[row] into *xrow when xrow is not null) are
-    // issued *before* the loads of the next tile: vector memory loads
-    // complete in issue order, so a gather issued after them would wait
-    // for the whole next tile.  The fetched registers are consumed by the
-    // caller's stage() in the same iteration, never carried across it.
+    // Row sum of the staged tile t, init + a_1 x_1 + a_2 x_2 + ... in column
+    // order (init = 0: the CSR row sum; init = y[row]: the COO product
+    // y(row) += a x entry by entry); starts fetching tile `next` (< 0:
+    // none) and loads the entry range of tile `after` (the tile following
+    // next; < 0: none).  The gathers of x (and x[row] into *xrow when xrow
+    // is not null) are issued first (see above).  The fetched registers are
+    // consumed by the caller's stage() in the same iteration, never carried
+    // across it.
     template <typename AT>
-    __device__ __forceinline__ AT rows(int n, int t, int next, const int* __restrict__ xadj,
-                                       const int* __restrict__ cols, const VT* __restrict__ vals,
-                                       const VT* __restrict__ x, const CsrStage<VT>& sm, int row,
-                                       VT* xrow)
-    {
-        return rows<AT>(n, t, next, xadj, cols, vals, x, sm, row, xrow, [] {});
-    }
-
-    // As above; pre() runs after this tile's gathers are issued and before
-    // the next tile's loads (for loads the caller needs for this tile).
-    template <typename AT, typename Pre>
-    __device__ __forceinline__ AT rows(int n, int t, int next, const int* __restrict__ xadj,
-                                       const int* __restrict__ cols, const VT* __restrict__ vals,
-                                       const VT* __restrict__ x, const CsrStage<VT>& sm, int row,
-                                       VT* xrow, Pre&& pre)
+    __device__ __forceinline__ AT rows(int n, int t, int next, int after,
+                                       const int* __restrict__ xadj, const int* __restrict__ cols,
+                                       const VT* __restrict__ vals, const VT* __restrict__ x,
+                                       const CsrStage<VT>& sm, int row, VT* xrow, AT init = AT(0))
     {
         const int b0 = e0, l = lo - e0, len = hi - lo;
         const bool staged = cnt <= kStageMax;  // uniform across the block
-        // branch-free gathers: entries past the row read a clamped LDS slot
-        // and x[0]; their products are not added
+        // Branch-free gathers and first kGather products: entries past the
+        // row read a clamped LDS slot and x[0], and a tile that is not staged
+        // reads stale slots (valid columns: start() clears them); those
+        // products are not kept.  With a branch on `staged` here the
+        // compiler moves loads into it, and the two paths then leave
+        // different loads in flight, so the waits below (for the next
+        // tile's entry range, for the gathers) would have to cover every
+        // load issued, the next tile's included.
         VT xv[kGather];
-        if (staged) {
 #pragma unroll
-            for (int i = 0; i < kGather; ++i) {
-                const int ci = sm.col[min(l + i, kStageCap - 1)];
-                xv[i] = x[i < len ? ci : 0];
-            }
+        for (int i = 0; i < kGather; ++i) {
+            const int ci = sm.col[min(l + i, kStageCap - 1)];
+            xv[i] = x[i < len ? ci : 0];
         }
         VT xr = VT(0);
         if (xrow) xr = x[row < n ? row : 0];
-        pre();
+        // keep the issue order (gathers, then the next tile): the scheduler
+        // would otherwise move the gathers next to their uses
+        __builtin_amdgcn_sched_barrier(0);
         fetch(n, next >= 0 ? next : t, next >= 0, xadj, cols, vals);
+        bounds(n, after, xadj);
+        __builtin_amdgcn_sched_barrier(0);
         if (xrow) *xrow = xr;
-        AT sum = AT(0);
-        if (staged) {
+        AT sum = init;
 #pragma unroll
-            for (int i = 0; i < kGather; ++i) {
-                const VT a = sm.val[min(l + i, kStageCap - 1)];
-                AT prod = AT(a) * AT(xv[i]);
-                AT tt = sum + prod;
-                sum = i < len ? tt : sum;
-            }
-            if (len > kGather)  // rows longer than kGather continue in column order
-                sum = row_sum_from<AT, VT, false>(sum, sm.val + l, sm.col + l, kGather, len, x);
-        } else {
-            sum = row_sum_from<AT, VT, true>(sum, vals + b0 + l, cols + b0 + l, 0, len, x);
+        for (int i = 0; i < kGather; ++i) {
+            const VT a = sm.val[min(l + i, kStageCap - 1)];
+            AT prod = AT(a) * AT(xv[i]);
+            AT tt = sum + prod;
+            sum = i < len ? tt : sum;
+        }
+        // Rows longer than kGather continue from LDS in column order; a tile
+        // that is not staged sums every row from global memory.  Both take
+        // the products above as their input (init for the second), so those
+        // stay needed on every path (a path on which they were dead would
+        // let the compiler move them, gathers included, into the other).
+        sum = staged ? sum : init;
+        const int j0 = staged ? min(len, kGather) : 0;
+        if (j0 < len) {
+            const VT* av = staged ? (const VT*)(sm.val + l) : vals + b0 + l;
+            const int* cv = staged ? (const int*)(sm.col + l) : cols + b0 + l;
+            sum = row_sum_from<AT, VT, false>(sum, av, cv, j0, len, x);
         }
         return sum;
     }
@@ -373,6 +425,99 @@ __device__ __forceinline__ AT jad_row(int k, int mmax, const int* __restrict__ b
     }
     return sum;
 }
+
+// JAD rows in the permuted row order, one per thread and 256-row tile,
+// software-pipelined like the CSR tile stream: row() issues the gathers of
+// the current row (its columns and values were loaded by the previous
+// call), then the loads of the next row, then sums.  The entry of permuted
+// row k on diagonal d sits at begin[d] + k, so every load is coalesced.
+// The lengths of the first kGather diagonals stay in scalar registers for
+// the whole walk; the columns, values and x go through one buffer resource
+// each, and an entry a row does not have is loaded from an offset outside
+// the resource (it reads 0, is gathered from x[0] and is not added).
+// Diagonals past kGather are summed after the first kGather, in order,
+// from global memory.  jad_nnz * sizeof(VT) and n * sizeof(VT) must stay
+// below 2^32 - 16 (unsigned 32-bit byte offsets).
+template <typename VT>
+struct JadStream {
+    int bg[kGather];  // starts of diagonals 0..kGather-1 (clamped)
+    int ln[kGather];  // their lengths (0 past the last diagonal)
+    int c[kGather];   // the fetched row's columns and values
+    VT a[kGather];
+    __amdgpu_buffer_rsrc_t rc, rv, rx;
+
+    __device__ __forceinline__ void init(int n, int mmax, long long nnz,
+                                         const int* __restrict__ begin, const int* cols,
+                                         const VT* vals, const VT* x)
+    {
+        constexpr unsigned B = sizeof(VT);
+        const int nd = min(mmax, kGather);
+#pragma unroll
+        for (int i = 0; i < kGather; ++i) {
+            bg[i] = __builtin_amdgcn_readfirstlane(begin[min(i, mmax)]);
+            const int e = __builtin_amdgcn_readfirstlane(begin[min(i + 1, mmax)]);
+            ln[i] = i < nd ? e - bg[i] : 0;
+        }
+        rc = __builtin_amdgcn_make_buffer_rsrc(const_cast<int*>(cols), (short)0,
+                                               (int)((unsigned)nnz * 4u), 0x00020000);
+        rv = __builtin_amdgcn_make_buffer_rsrc(const_cast<VT*>(vals), (short)0,
+                                               (int)((unsigned)nnz * B), 0x00020000);
+        rx = __builtin_amdgcn_make_buffer_rsrc(const_cast<VT*>(x), (short)0,
+                                               (int)((unsigned)n * B), 0x00020000);
+    }
+
+    // every load is issued (entries a row lacks read from outside the
+    // resources), so the loads in flight do not depend on the data and the
+    // waits after them stay exact
+    __device__ __forceinline__ void fetch(int k)
+    {
+        constexpr unsigned B = sizeof(VT);
+#pragma unroll
+        for (int i = 0; i < kGather; ++i) {
+            const unsigned e = (unsigned)(bg[i] + k);
+            const bool on = k < ln[i];
+            c[i] = (int)__builtin_amdgcn_raw_buffer_load_b32(rc, on ? e * 4u : 0xFFFFFFFFu, 0, 2);
+            a[i] = buf_load_nt<VT>(rv, (int)(on ? e * B : 0xFFFFFFFFu), 0);
+        }
+    }
+
+    // Sum of permuted row k (fetched by the previous call) in diagonal
+    // order; fetches row kn (kn >= n: loads that return 0) and returns
+    // x[k] (0 for k >= n) in xr.
+    template <typename AT>
+    __device__ __forceinline__ AT row(int k, int kn, int mmax, const int* __restrict__ begin,
+                                      const int* __restrict__ cols, const VT* __restrict__ vals,
+                                      const VT* __restrict__ x, VT& xr)
+    {
+        constexpr unsigned B = sizeof(VT);
+        VT xv[kGather];
+#pragma unroll
+        for (int i = 0; i < kGather; ++i) xv[i] = buf_load<VT>(rx, (int)((unsigned)c[i] * B));
+        xr = buf_load<VT>(rx, (int)((unsigned)k * B));
+        VT ac[kGather];
+#pragma unroll
+        for (int i = 0; i < kGather; ++i) ac[i] = a[i];
+        __builtin_amdgcn_sched_barrier(0);
+        fetch(kn);
+        __builtin_amdgcn_sched_barrier(0);
+        AT sum = AT(0);
+#pragma unroll
+        for (int i = 0; i < kGather; ++i) {
+            AT prod = AT(ac[i]) * AT(xv[i]);
+            AT tt = sum + prod;
+            sum = k < ln[i] ? tt : sum;
+        }
+        if (mmax > kGather) {  // uniform; rare (more than kGather diagonals)
+            for (int d = kGather; d < mmax; ++d) {
+                const int b = begin[d];
+                if (k >= begin[d + 1] - b) break;
+                AT prod = AT(ld_stream(vals + b + k)) * AT(x[ld_stream(cols + b + k)]);
+                sum = sum + prod;
+            }
+        }
+        return sum;
+    }
+};
 
 // Halo entries (rows of this rank with columns owned by a neighbour).
 // hrow_tile[t]..hrow_tile[t+1] index the compact, ascending list of rows of

@@ -72,6 +72,16 @@ def test_dia_refuses_what_it_cannot_hold(gpu):
     assert e.value.code == gpu.ERR_BAD_MATRIX
     with pytest.raises(gpu.CelError):
         M.spmv(np.ones(n), fmt=gpu.FMT_DIA)
+    # a rectangular 4 x 8 block (an entry in column 6): the DIA row sums
+    # read x as n entries, so the block has no DIA form; its CSR product
+    # stays available
+    R = gpu.Matrix.from_csr([1, 3, 4, 6, 7], [1, 7, 2, 3, 4, 4], [1.0, 2.0, 3.0, 4.0, 5.0, 6.0],
+                            ncols=8)
+    with pytest.raises(gpu.CelError) as e:
+        R.to_dia()
+    assert e.value.code == gpu.ERR_BAD_MATRIX
+    x = np.arange(1.0, 9.0)
+    np.testing.assert_array_equal(R.spmv(x), [1 + 14, 6, 12 + 20, 24])
 
 
 def test_scaling_drops_the_dia_form(gpu):
@@ -137,23 +147,3 @@ def test_pcg_dia_with_halo_faces_is_bitwise_the_csr_solve(gpu):
     # the halo path is FP64 (CSR or DIA form)
     with pytest.raises(gpu.CelError):
         M.pcg(b, use_halo=1, fmt=gpu.FMT_DIA, precision=gpu.PREC_MIXED, **kw)
-
-
-def test_pcg_dia_256_timing(gpu):
-    """The 256^3 bench block: CSR and DIA mixed solves agree bit for bit;
-    prints both times and the K1 means."""
-    M = gpu.Matrix.poisson7(256, 256, 256)
-    M.set_jacobi()
-    xt = np.random.default_rng(1).random(M.n)
-    d_xt = gpu.DeviceArray.from_host(xt)
-    d_b = gpu.DeviceArray(M.n)
-    M.spmv(d_xt, d_b)
-    kw = dict(tol=1e-10, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, precision=gpu.PREC_MIXED,
-              iter_max=20000)
-    a = M.pcg(d_b, **kw)
-    M.to_dia()
-    d = M.pcg(d_b, fmt=gpu.FMT_DIA, **kw)
-    d = M.pcg(d_b, fmt=gpu.FMT_DIA, **kw)
-    print(f"mixed CSR {a['solve_ms']:.1f} ms, DIA {d['solve_ms']:.1f} ms, iters {a['iters']}/{d['iters']}",
-          flush=True)
-    np.testing.assert_array_equal(a["x"].to_host(), d["x"].to_host())

@@ -80,6 +80,16 @@ def test_split_pack_and_distributed_spmv(gpu, name, nblocks):
         np.testing.assert_array_equal(y, yo)
         mats[r].to_jad()
         np.testing.assert_array_equal(mats[r].spmv_distr(xs[r], ghost, fmt=gpu.FMT_JAD), yo)
+        # the local block as COO (mv_algorithm = MV_COO): every block's
+        # entries added to y = 0 one by one, local block first
+        yc = O.spmv_coo(b0["rows"], b0["cols"], b0["vals"], ref["row_offset"], b0["col_offset"],
+                        xs[r], np.zeros(xs[r].size))
+        for rb in ref["blocks"][1:]:
+            q = rb["proc"]
+            xn = np.zeros(vtx[q + 1] - vtx[q])
+            xn[rb["comm_perm"] - 1] = xs[q][rb["comm_perm"] - 1]
+            yc = O.spmv_coo(rb["rows"], rb["cols"], rb["vals"], ref["row_offset"], rb["col_offset"], xn, yc)
+        np.testing.assert_array_equal(mats[r].spmv_distr(xs[r], ghost, fmt=gpu.FMT_COO), yc)
 
 
 def block_major_poisson(m, d, rank):

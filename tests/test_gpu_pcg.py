@@ -35,10 +35,12 @@ def compare(ref, got, n_hist=None):
     return rel.max()
 
 
-@pytest.mark.parametrize("fmt", ["csr", "jad"])
+@pytest.mark.parametrize("fmt", ["csr", "jad", "coo"])
 def test_config1_poisson32_parity(gpu, fmt):
     """32^3, x_true ~ U[0,1) seed 0, x0 = 0, relative tolerance 1e-8,
-    iter_max 1000, reference with 4 threads (3 workers)."""
+    iter_max 1000, reference with 4 threads (3 workers).  COO applies the
+    block entry by entry (v(row) += a p from v = 0), which gives the CSR
+    solve's bits."""
     A, xt, b, jac = problem(32, 0)
     eps = 1e-8 * np.linalg.norm(b)
     ref = O.pcg_jacobi(A, jac, b, x0_mode=O.X0_ZERO, eps=eps, iter_max=1000, workers=3)
@@ -48,8 +50,10 @@ def test_config1_poisson32_parity(gpu, fmt):
     if fmt == "jad":
         M.to_jad()
         f = gpu.FMT_JAD
-    got = M.pcg(b, tol=1e-8, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, iter_max=1000, fmt=f,
-                history=True)
+    elif fmt == "coo":
+        f = gpu.FMT_COO
+    kw = dict(tol=1e-8, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, iter_max=1000, history=True)
+    got = M.pcg(b, fmt=f, **kw)
     print(f"iters gpu {got['iters']} ref {ref['iters']}", flush=True)
     assert got["status"] == 0
     rel = compare(ref, got)
@@ -57,6 +61,13 @@ def test_config1_poisson32_parity(gpu, fmt):
     np.testing.assert_allclose(got["x"], ref["x"], rtol=0, atol=1e-9)
     assert np.abs(got["x"] - xt).max() < 1e-6
     assert got["final_norm"] <= got["eps_abs"]
+    if fmt == "coo":
+        csr = M.pcg(b, fmt=gpu.FMT_CSR, **kw)
+        np.testing.assert_array_equal(got["hist"], csr["hist"])
+        np.testing.assert_array_equal(got["x"], csr["x"])
+        mc = M.pcg(b, fmt=f, precision=gpu.PREC_MIXED, **kw)
+        mr = M.pcg(b, fmt=gpu.FMT_CSR, precision=gpu.PREC_MIXED, **kw)
+        np.testing.assert_array_equal(mc["x"], mr["x"])
 
 
 def test_reference_defaults_x0_one_absolute_eps(gpu):

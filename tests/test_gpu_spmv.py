@@ -34,6 +34,26 @@ def test_csr_spmv_bitwise(gpu, name, A):
 
 
 @pytest.mark.parametrize("name,A", CASES, ids=[c[0] for c in CASES])
+def test_coo_spmv_bitwise(gpu, name, A):
+    """CEL_FMT_COO (cel_fop_mv_module.f90:357-403): y(row) = y(row) + a x for
+    every entry in the block's row-sorted order, from y = 0 or, when
+    accumulating, from the caller's y; bitwise the oracle's entry loop."""
+    M = dev_matrix(gpu, A)
+    rows = np.repeat(np.arange(1, A.n + 1), np.diff(A.xadj))
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=A.n)
+    np.testing.assert_array_equal(M.spmv(x, fmt=gpu.FMT_COO),
+                                  O.spmv_coo(rows, A.cols, A.vals, 0, 0, x, np.zeros(A.n)))
+    y0 = 1e3 * rng.normal(size=A.n)
+    yc = O.spmv_coo(rows, A.cols, A.vals, 0, 0, x, y0)
+    np.testing.assert_array_equal(M.spmv(x, y0, fmt=gpu.FMT_COO, accumulate=True), yc)
+    # from 0 the entry-by-entry sum is the CSR row sum; FP32 as well
+    np.testing.assert_array_equal(M.spmv(x, fmt=gpu.FMT_COO), M.spmv(x))
+    xf = x.astype(np.float32)
+    np.testing.assert_array_equal(M.spmv_f32(xf, fmt=gpu.FMT_COO), O.spmv_csr_f32(A, xf))
+
+
+@pytest.mark.parametrize("name,A", CASES, ids=[c[0] for c in CASES])
 def test_csr_spmv_f32_bitwise(gpu, name, A):
     M = dev_matrix(gpu, A)
     x = np.random.default_rng(2).normal(size=A.n).astype(np.float32)

@@ -783,3 +783,15 @@ def test_gmres_final_norm_is_the_true_residual_and_shows_the_rotation_defect():
             assert ref["norm"] <= 1.01e-8
         else:
             assert ref["norm"] > 100 * 1e-8
+
+
+def test_coo_accumulate_adds_entry_by_entry():
+    """The COO product into a non-zero y is not the CSR one: y + (a1 x1 +
+    a2 x2) rounds differently from (y + a1 x1) + a2 x2.  (Pins that the
+    accumulating COO case of the GPU tests checks the entry order.)"""
+    A = O.CSR(np.array([1, 3]), np.array([1, 2]), np.array([1.0, 1.0]))
+    x = np.array([1.0, 1e-16])
+    y0 = np.array([-1.0])
+    coo = O.spmv_coo(np.array([1, 1]), A.cols, A.vals, 0, 0, x, y0)
+    csr = O.spmv_csr(A, x, y0)
+    assert coo[0] == 1e-16 and csr[0] == 0.0
