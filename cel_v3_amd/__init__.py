@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libcel_gpu.so")
 OK, NOT_CONVERGED = 0, 1
 ERR = {-1: "ARG", -2: "HIP", -3: "NO_DEVICE", -4: "BAD_MATRIX", -5: "BAD_DIAG", -6: "BREAKDOWN",
        -7: "NONFINITE", -8: "OOM", -9: "REF_UNDEFINED", -10: "STATE", -11: "FILE"}
-ERR_BAD_MATRIX, ERR_REF_UNDEFINED, ERR_FILE = -4, -9, -11
+ERR_ARG, ERR_BAD_MATRIX, ERR_REF_UNDEFINED, ERR_FILE = -1, -4, -9, -11
 FMT_CSR, FMT_JAD, FMT_COO, FMT_DIA = 1, 2, 3, 4
 MEM_HOST, MEM_DEVICE = 0, 1
 X0_ONE, X0_ZERO, X0_USER = 0, 1, 2
@@ -53,6 +53,7 @@ EXPORTS = [
     "cel_gpu_elm_row_completed", "cel_gpu_elm_matrix_completed", "cel_gpu_elm_set_solver",
     "cel_gpu_elm_solve", "cel_gpu_elm_resolve", "cel_gpu_elm_collect_solution", "cel_gpu_elm_stats",
     "cel_gpu_elm_matrix", "cel_gpu_elm_destroy", "cel_gpu_mat_to_dia", "cel_gpu_mat_dia_info",
+    "cel_gpu_pcg_solve_noalloc",
 ]
 
 
@@ -431,6 +432,19 @@ class Matrix:
         if history:
             out["hist"] = hist[: st.iters + 1]
         return out
+
+    def pcg_noalloc(self, b: "DeviceArray", x: "DeviceArray", tol=1e-8, tol_mode=TOL_ABS,
+                    iter_max=1000, x0_mode=X0_ONE, fmt=FMT_CSR, precision=PREC_FP64,
+                    allow_not_converged=True, **kw):
+        """cel_gpu_pcg_solve_noalloc: the FP64 solve without allocations, b and
+        x device arrays (x holds x0 with X0_USER and receives the solution)."""
+        p = params(tol=tol, tol_mode=tol_mode, iter_max=iter_max, x0_mode=x0_mode,
+                   precision=precision, format=fmt, **kw)
+        p.memspace = MEM_DEVICE
+        st = PcgStats()
+        rc = lib().cel_gpu_pcg_solve_noalloc(self.h, b.ptr, x.ptr, C.byref(p), C.byref(st))
+        _check(rc, allow=(OK, NOT_CONVERGED) if allow_not_converged else (OK,))
+        return st.as_dict()
 
     def gmres(self, b, x0=None, eps=1e-8, iter_max=5000, restart=3, x0_mode=X0_ONE, fmt=FMT_CSR,
               allow_not_converged=True):

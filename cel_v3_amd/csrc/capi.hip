@@ -12,7 +12,8 @@
 namespace cel {
 
 int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel_pcg_stats& stats,
-              double* hist);
+              double* hist, bool noalloc = false);
+int pcg_noalloc_setup(int device);
 
 static thread_local std::string g_err;
 
@@ -71,7 +72,8 @@ extern "C" int cel_gpu_init(int device)
     R.device = device;
     R.num_cus = prop.multiProcessorCount;
     R.ready = true;
-    return CEL_OK;
+    // the allocation-free solve's scratch address on this device
+    return pcg_noalloc_setup(device);
 }
 
 extern "C" int cel_gpu_finalize(void)
@@ -948,6 +950,17 @@ extern "C" int cel_gpu_pcg_solve(cel_gpu_mat* A, const double* b, double* x,
     cel_pcg_stats st;
     std::memset(&st, 0, sizeof(st));
     int rc = pcg_solve(A->m, b, x, p, st, hist);
+    if (stats) *stats = st;
+    return rc;
+}
+
+extern "C" int cel_gpu_pcg_solve_noalloc(cel_gpu_mat* A, const double* b, double* x,
+                                         const cel_pcg_params* params, cel_pcg_stats* stats)
+{
+    if (!A || !b || !x || !params) return fail(CEL_ERR_ARG, "null argument");
+    cel_pcg_stats st;
+    std::memset(&st, 0, sizeof(st));
+    int rc = pcg_solve(A->m, b, x, *params, st, nullptr, true);
     if (stats) *stats = st;
     return rc;
 }

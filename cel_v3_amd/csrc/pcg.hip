@@ -33,6 +33,7 @@
 // add, -ffp-contract=off); only the summation order of the dot products
 // differs from the reference (fixed tree here, worker chunks there).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <initializer_list>
 #include <vector>
@@ -808,8 +809,40 @@ static int common_grid(std::initializer_list<const void*> kernels, long long wor
     return g;
 }
 
+// The allocation-free entry point's device memory: a file-scope array
+// (one copy per GPU, resolved to a device address once per device), laid
+// out as r, p, v (kNoAllocMaxRows doubles each), the three partial-sum
+// arrays (kNoAllocMaxGrid, 2x and 3x that) and the five token slots.
+constexpr size_t kNoAllocMaxRows = size_t(1) << 20;
+constexpr size_t kNoAllocMaxGrid = 8192;
+constexpr size_t kNoAllocDoubles = 3 * kNoAllocMaxRows + 6 * kNoAllocMaxGrid;
+constexpr size_t kNoAllocBytes = kNoAllocDoubles * sizeof(double) + 5 * 128;
+static_assert(sizeof(Ctrl) <= 128, "token slot");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+constexpr int kMaxDevices = 64;
+static void* g_scratch[kMaxDevices];
+
+int pcg_noalloc_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(CEL_ERR_ARG, "device index beyond the scratch table");
+    if (g_scratch[device]) return CEL_OK;
+    void* ptr = nullptr;
+    // the name `scratch` is a host-side stand-in; kernels get this address
+    CEL_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[device] = ptr;
+    return CEL_OK;
+}
+
+// Device pointers of one solve's work vectors, partial sums and token.
+struct WorkPtrs {
+    double *r = nullptr, *p = nullptr, *v = nullptr, *part1 = nullptr, *part2 = nullptr,
+           *rpart = nullptr;
+    float *r32 = nullptr, *p32 = nullptr, *v32 = nullptr, *e32 = nullptr;
+    Ctrl* ctrl = nullptr;
+};
+
 int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel_pcg_stats& stats,
-              double* hist)
+              double* hist, bool noalloc)
 {
     CEL_TRY(ensure_init());
     hipStream_t s = rt().stream;
@@ -831,6 +864,19 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     if (!A.has_jac) return fail(CEL_ERR_STATE, "cel_gpu_mat_set_jacobi has not been called");
     if (A.ncols != n) return fail(CEL_ERR_ARG, "PCG needs a square local block");
     if (prm.iter_max < 0) return fail(CEL_ERR_ARG, "iter_max < 0");
+    if (noalloc) {
+        if (mixed || halo || fmt == CEL_FMT_JAD)
+            return fail(CEL_ERR_ARG, "the allocation-free solve runs FP64 in the CSR, COO or DIA form, "
+                                     "without halo blocks");
+        if (prm.memspace != CEL_MEM_DEVICE)
+            return fail(CEL_ERR_ARG, "the allocation-free solve takes device pointers");
+        if (hist) return fail(CEL_ERR_ARG, "the allocation-free solve records no history");
+        if ((size_t)n > kNoAllocMaxRows)
+            return fail(CEL_ERR_ARG, "the allocation-free solve holds at most " +
+                                         std::to_string(kNoAllocMaxRows) + " rows");
+        if (rt().device < 0 || rt().device >= kMaxDevices || !g_scratch[rt().device])
+            return fail(CEL_ERR_STATE, "no scratch address for this device (cel_gpu_init)");
+    }
     if (n == 0) {  // nothing to solve (the kernels assume at least one row)
         stats = cel_pcg_stats{};
         stats.status = CEL_OK;
@@ -848,7 +894,6 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         CEL_TRY(jad_prepare_permuted(A));
     }
 
-    Workspace& W = ws_of(A);
     const int ntiles = std::max(1, (n + kTileRows - 1) / kTileRows);
     // grids: K1 variants that must agree (with and without halo) share one
     // grid; the residual kernels share one; K2 and K3 share one.
@@ -891,6 +936,11 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         if (Gv >= kXcds) Gv = Gv / kXcds * kXcds;
     }
 
+    if (noalloc && ((size_t)G1 > kNoAllocMaxGrid || (size_t)Gr > kNoAllocMaxGrid ||
+                    (size_t)Gv > kNoAllocMaxGrid))
+        return fail(CEL_ERR_ARG, "the allocation-free solve holds at most " +
+                                     std::to_string(kNoAllocMaxGrid) + " blocks per kernel");
+
     // matrix data for the mixed inner operator (once per matrix)
     if (mixed) {
         if (fmt == CEL_FMT_DIA) CEL_TRY(dia_prepare_f32(A));
@@ -906,25 +956,51 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         }
     }
 
-    // workspace
+    // workspace: the matrix handle's buffers, or the scratch array
     const size_t nn = n > 0 ? n : 1;
-    CEL_TRY(W.part1.alloc((size_t)G1));
-    CEL_TRY(W.part2.alloc(2 * (size_t)Gv));
-    CEL_TRY(W.rpart.alloc(3 * (size_t)Gr));
-    CEL_TRY(W.ctrl.alloc(5));
-    if (mixed) {
-        CEL_TRY(W.r32.alloc(nn));
-        CEL_TRY(W.p32.alloc(nn));
-        CEL_TRY(W.v32.alloc(nn));
-        CEL_TRY(W.e32.alloc(nn));
+    Workspace* Wp = noalloc ? nullptr : &ws_of(A);
+    WorkPtrs w;
+    if (noalloc) {
+        double* base = static_cast<double*>(g_scratch[rt().device]);
+        w.r = base;
+        w.p = base + kNoAllocMaxRows;
+        w.v = base + 2 * kNoAllocMaxRows;
+        w.part1 = base + 3 * kNoAllocMaxRows;
+        w.part2 = w.part1 + kNoAllocMaxGrid;
+        w.rpart = w.part2 + 2 * kNoAllocMaxGrid;
+        w.ctrl = reinterpret_cast<Ctrl*>(base + kNoAllocDoubles);
     } else {
-        CEL_TRY(W.r.alloc(nn));
-        CEL_TRY(W.p.alloc(nn));
-        CEL_TRY(W.v.alloc(nn));
+        Workspace& W = *Wp;
+        CEL_TRY(W.part1.alloc((size_t)G1));
+        CEL_TRY(W.part2.alloc(2 * (size_t)Gv));
+        CEL_TRY(W.rpart.alloc(3 * (size_t)Gr));
+        CEL_TRY(W.ctrl.alloc(5));
+        if (mixed) {
+            CEL_TRY(W.r32.alloc(nn));
+            CEL_TRY(W.p32.alloc(nn));
+            CEL_TRY(W.v32.alloc(nn));
+            CEL_TRY(W.e32.alloc(nn));
+        } else {
+            CEL_TRY(W.r.alloc(nn));
+            CEL_TRY(W.p.alloc(nn));
+            CEL_TRY(W.v.alloc(nn));
+        }
+        w.r = W.r.p;
+        w.p = W.p.p;
+        w.v = W.v.p;
+        w.part1 = W.part1.p;
+        w.part2 = W.part2.p;
+        w.rpart = W.rpart.p;
+        w.ctrl = W.ctrl.p;
+        w.r32 = W.r32.p;
+        w.p32 = W.p32.p;
+        w.v32 = W.v32.p;
+        w.e32 = W.e32.p;
     }
     const double* db = b;
     double* dx = x;
     if (prm.memspace == CEL_MEM_HOST) {
+        Workspace& W = *Wp;
         CEL_TRY(W.b.alloc(nn));
         CEL_TRY(W.x.alloc(nn));
         CEL_HIP(hipMemcpyAsync(W.b.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
@@ -937,6 +1013,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     const double* jac64 = A.jac.p;
     const float* jac32 = A.jac32.p;
     if (jadp) {
+        Workspace& W = *Wp;
         CEL_TRY(W.bp.alloc(nn));
         CEL_TRY(W.xp.alloc(nn));
         CEL_TRY(W.jacp.alloc(nn));
@@ -945,8 +1022,9 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         jac32 = mixed ? W.jacp32.p : nullptr;
     }
     const int hist_cap = hist ? prm.iter_max + 1 : 0;
-    if (hist_cap) CEL_TRY(W.hist.alloc(hist_cap));
+    if (hist_cap) CEL_TRY(Wp->hist.alloc(hist_cap));
     if (halo) {
+        Workspace& W = *Wp;
         CEL_TRY(W.ghost.alloc(A.ghost_len > 0 ? A.ghost_len : 1));
         CEL_TRY(W.sendbuf.alloc(A.send_total > 0 ? A.send_total : 1));
         hipLaunchKernelGGL(k_fill, dim3(64), dim3(kBlock), 0, s, (long long)W.ghost.n, W.ghost.p, 0.0);
@@ -964,17 +1042,19 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     P.ru_restart = prm.ru_restart;
     P.alpha_stop = prm.alpha_stop;
     P.ru_max_inner = 1000;
-    P.hist = hist_cap ? W.hist.p : nullptr;
+    P.hist = hist_cap ? Wp->hist.p : nullptr;
     P.hist_cap = hist_cap;
     Ctrl c0{};
     c0.run = 1;
-    CEL_HIP(hipMemcpyAsync(W.ctrl.p, &c0, sizeof(Ctrl), hipMemcpyHostToDevice, s));
+    // every solve starts from a fresh token (the scratch array keeps the
+    // last call's contents; everything else is written before it is read)
+    CEL_HIP(hipMemcpyAsync(w.ctrl, &c0, sizeof(Ctrl), hipMemcpyHostToDevice, s));
     // token slots: 0 -> K1 -> 1 -> K2 -> 2 -> K3 -> 0 (FP64);
     // mixed: K3 -> 3 -> replacement residual -> 0 (K3 folds e into x64)
-    Ctrl* C0 = W.ctrl.p;
-    Ctrl* C1 = W.ctrl.p + 1;
-    Ctrl* C2 = W.ctrl.p + 2;
-    Ctrl* C3 = W.ctrl.p + 3;
+    Ctrl* C0 = w.ctrl;
+    Ctrl* C1 = w.ctrl + 1;
+    Ctrl* C2 = w.ctrl + 2;
+    Ctrl* C3 = w.ctrl + 3;
 
     MatView V{};
     V.n = n;
@@ -1002,22 +1082,28 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         if (A.dia_off[d] == 0) V.dzero = d;
     }
     if (halo) {
-        V.hrow_tile = W.hrow_tile.p;
+        V.hrow_tile = Wp->hrow_tile.p;
         V.hrows = A.halo_rows.p;
         V.hrowq = A.halo_rowq.p;
         V.hptr = A.halo_ptr.p;
         V.hcols = A.halo_cols.p;
         V.hvals = A.halo_vals.p;
-        V.ghost = W.ghost.p;
+        V.ghost = Wp->ghost.p;
     }
 
+    // the allocation-free solve creates no events: it times with the host
+    // clock around a synchronised stream
     StreamScope R(s);
-    hipEvent_t e0, e1, e2;
-    CEL_TRY(R.event(e0));
-    CEL_TRY(R.event(e1));
-    CEL_TRY(R.event(e2));
-    CEL_HIP(hipEventRecord(e0, s));
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    const auto t_host0 = std::chrono::steady_clock::now();
+    if (!noalloc) {
+        CEL_TRY(R.event(e0));
+        CEL_TRY(R.event(e1));
+        CEL_TRY(R.event(e2));
+        CEL_HIP(hipEventRecord(e0, s));
+    }
     if (jadp) {
+        Workspace& W = *Wp;
         const int gp = std::max(1, std::min(Gv * 2, 4096));
         hipLaunchKernelGGL(k_perm_gather<double>, dim3(gp), dim3(kBlock), 0, s, (long long)n,
                            A.jad_perm.p, db, W.bp.p);
@@ -1038,33 +1124,33 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     if (prm.x0_mode != CEL_X0_USER)
         hipLaunchKernelGGL(k_fill, dim3(std::max(1, std::min(Gv * 2, 4096))), dim3(kBlock), 0, s,
                            (long long)n, dx, prm.x0_mode == CEL_X0_ONE ? 1.0 : 0.0);
-    if (halo) CEL_TRY(launch_face_pack(A, dx, W.sendbuf.p));
+    if (halo) CEL_TRY(launch_face_pack(A, dx, Wp->sendbuf.p));
 
     // r0, p0 and their partials; the first K1 folds them in.  The set-up
     // reads and writes token slot 0 (only block 0 writes, after its read).
 #define RES_ARGS(OT, R, P_)                                                                \
-    V, dx, db, jac64, jac32, (OT*)(R), (OT*)(P_), W.e32.p, W.rpart.p, C0, C0, P
+    V, dx, db, jac64, jac32, (OT*)(R), (OT*)(P_), w.e32, w.rpart, C0, C0, P
     if (!mixed) {
         if (rfmt == CEL_FMT_CSR && halo)
             hipLaunchKernelGGL((k_residual<double, CEL_FMT_CSR, true, false>), dim3(Gr), dim3(kBlock), 0, s,
-                               RES_ARGS(double, W.r.p, W.p.p));
+                               RES_ARGS(double, w.r, w.p));
         else if (rfmt == CEL_FMT_CSR)
             hipLaunchKernelGGL((k_residual<double, CEL_FMT_CSR, false, false>), dim3(Gr), dim3(kBlock), 0,
-                               s, RES_ARGS(double, W.r.p, W.p.p));
+                               s, RES_ARGS(double, w.r, w.p));
         else
             hipLaunchKernelGGL((k_residual<double, CEL_FMT_JAD, false, false>), dim3(Gr), dim3(kBlock), 0,
-                               s, RES_ARGS(double, W.r.p, W.p.p));
+                               s, RES_ARGS(double, w.r, w.p));
     } else {
         if (rfmt == CEL_FMT_CSR)
             hipLaunchKernelGGL((k_residual<float, CEL_FMT_CSR, false, false>), dim3(Gr), dim3(kBlock), 0, s,
-                               RES_ARGS(float, W.r32.p, W.p32.p));
+                               RES_ARGS(float, w.r32, w.p32));
         else
             hipLaunchKernelGGL((k_residual<float, CEL_FMT_JAD, false, false>), dim3(Gr), dim3(kBlock), 0, s,
-                               RES_ARGS(float, W.r32.p, W.p32.p));
+                               RES_ARGS(float, w.r32, w.p32));
     }
 #undef RES_ARGS
     CEL_HIP(hipGetLastError());
-    CEL_HIP(hipEventRecord(e1, s));
+    if (!noalloc) CEL_HIP(hipEventRecord(e1, s));
 
     // profiling events around K1
     std::vector<hipEvent_t> pev;
@@ -1081,7 +1167,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     const long long n_ll = n;
     auto enqueue_iter = [&](bool profile) -> int {
         hipEvent_t ea = nullptr, eb = nullptr;
-        if (halo) CEL_TRY(launch_face_pack(A, W.p.p, W.sendbuf.p));
+        if (halo) CEL_TRY(launch_face_pack(A, w.p, Wp->sendbuf.p));
         if (profile) {
             ea = prof_event();
             eb = prof_event();
@@ -1091,46 +1177,46 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         if (!mixed) {
             if (halo && fmt == CEL_FMT_DIA)
                 hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_DIA, true>), dim3(G1), dim3(kBlock), 0, s, V,
-                                   W.p.p, W.v.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
+                                   w.p, w.v, w.part1, w.rpart, Gr, C0, C1, P);
             else if (halo)
                 hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_CSR, true>), dim3(G1), dim3(kBlock), 0, s, V,
-                                   W.p.p, W.v.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
+                                   w.p, w.v, w.part1, w.rpart, Gr, C0, C1, P);
             else if (fmt == CEL_FMT_CSR)
                 hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_CSR, false>), dim3(G1), dim3(kBlock), 0, s, V,
-                                   W.p.p, W.v.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
+                                   w.p, w.v, w.part1, w.rpart, Gr, C0, C1, P);
             else if (fmt == CEL_FMT_DIA)
                 hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_DIA, false>), dim3(G1), dim3(kBlock), 0, s, V,
-                                   W.p.p, W.v.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
+                                   w.p, w.v, w.part1, w.rpart, Gr, C0, C1, P);
             else
                 hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_JAD, false>), dim3(G1), dim3(kBlock), 0, s, V,
-                                   W.p.p, W.v.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
+                                   w.p, w.v, w.part1, w.rpart, Gr, C0, C1, P);
             if (profile) CEL_HIP(hipEventRecord(eb, s));
-            hipLaunchKernelGGL((k_pcg_update_r<double>), dim3(Gv), dim3(kBlock), 0, s, n_ll, W.r.p, W.v.p,
-                               jac64, W.part2.p, W.part1.p, G1, C1, C2);
+            hipLaunchKernelGGL((k_pcg_update_r<double>), dim3(Gv), dim3(kBlock), 0, s, n_ll, w.r, w.v,
+                               jac64, w.part2, w.part1, G1, C1, C2);
             hipLaunchKernelGGL((k_pcg_update_xp<double, double, false>), dim3(Gv), dim3(kBlock), 0, s, n_ll,
-                               dx, W.p.p, W.r.p, jac64, nullptr, W.part2.p, Gv, C2, C0, P);
+                               dx, w.p, w.r, jac64, nullptr, w.part2, Gv, C2, C0, P);
         } else {
             if (fmt == CEL_FMT_CSR)
                 hipLaunchKernelGGL((k_pcg_spmv<float, CEL_FMT_CSR, false>), dim3(G1), dim3(kBlock), 0, s, V,
-                                   W.p32.p, W.v32.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
+                                   w.p32, w.v32, w.part1, w.rpart, Gr, C0, C1, P);
             else if (fmt == CEL_FMT_DIA)
                 hipLaunchKernelGGL((k_pcg_spmv<float, CEL_FMT_DIA, false>), dim3(G1), dim3(kBlock), 0, s, V,
-                                   W.p32.p, W.v32.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
+                                   w.p32, w.v32, w.part1, w.rpart, Gr, C0, C1, P);
             else
                 hipLaunchKernelGGL((k_pcg_spmv<float, CEL_FMT_JAD, false>), dim3(G1), dim3(kBlock), 0, s, V,
-                                   W.p32.p, W.v32.p, W.part1.p, W.rpart.p, Gr, C0, C1, P);
+                                   w.p32, w.v32, w.part1, w.rpart, Gr, C0, C1, P);
             if (profile) CEL_HIP(hipEventRecord(eb, s));
-            hipLaunchKernelGGL((k_pcg_update_r<float>), dim3(Gv), dim3(kBlock), 0, s, n_ll, W.r32.p, W.v32.p,
-                               jac32, W.part2.p, W.part1.p, G1, C1, C2);
+            hipLaunchKernelGGL((k_pcg_update_r<float>), dim3(Gv), dim3(kBlock), 0, s, n_ll, w.r32, w.v32,
+                               jac32, w.part2, w.part1, G1, C1, C2);
             hipLaunchKernelGGL((k_pcg_update_xp<float, float, true>), dim3(Gv), dim3(kBlock), 0, s, n_ll,
-                               W.e32.p, W.p32.p, W.r32.p, jac32, dx, W.part2.p, Gv, C2, C3, P);
+                               w.e32, w.p32, w.r32, jac32, dx, w.part2, Gv, C2, C3, P);
             if (rfmt == CEL_FMT_CSR)
                 hipLaunchKernelGGL((k_residual<float, CEL_FMT_CSR, false, true>), dim3(Gr), dim3(kBlock), 0, s,
-                                   V, dx, db, jac64, jac32, W.r32.p, W.p32.p, W.e32.p, W.rpart.p, C3,
+                                   V, dx, db, jac64, jac32, w.r32, w.p32, w.e32, w.rpart, C3,
                                    C0, P);
             else
                 hipLaunchKernelGGL((k_residual<float, CEL_FMT_JAD, false, true>), dim3(Gr), dim3(kBlock), 0, s,
-                                   V, dx, db, jac64, jac32, W.r32.p, W.p32.p, W.e32.p, W.rpart.p, C3,
+                                   V, dx, db, jac64, jac32, w.r32, w.p32, w.e32, w.rpart, C3,
                                    C0, P);
         }
         CEL_HIP(hipGetLastError());
@@ -1142,13 +1228,16 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     // iter_max iterations plus one K1 that folds in the final partials.
     const int chunk = prm.check_every > 0 ? prm.check_every : 16;
     int* hrun = nullptr;
-    CEL_TRY(R.host_alloc(hrun, 2));
-    hrun[0] = hrun[1] = 1;
-    hipEvent_t cev[2];
-    CEL_TRY(R.event(cev[0], hipEventDisableTiming));
-    CEL_TRY(R.event(cev[1], hipEventDisableTiming));
+    int hrun_sync = 1;  // the allocation-free solve reads the flag synchronously
+    hipEvent_t cev[2] = {nullptr, nullptr};
+    if (!noalloc) {
+        CEL_TRY(R.host_alloc(hrun, 2));
+        hrun[0] = hrun[1] = 1;
+        CEL_TRY(R.event(cev[0], hipEventDisableTiming));
+        CEL_TRY(R.event(cev[1], hipEventDisableTiming));
+    }
 
-    const bool use_graph = prm.use_graph && !prm.profile;
+    const bool use_graph = prm.use_graph && !prm.profile && !noalloc;
     int status = CEL_OK;
     const long long budget = (long long)prm.iter_max + 1;
     long long enqueued = 0;
@@ -1171,12 +1260,19 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
             CEL_HIP(hipGraphLaunch(R.exec, s));
         } else {
             for (int i = 0; i < c; ++i) {
-                status = enqueue_iter(prm.profile != 0);
+                status = enqueue_iter(prm.profile != 0 && !noalloc);
                 if (status < 0) break;
             }
             if (status < 0) break;
         }
         enqueued += c;
+        if (noalloc) {
+            CEL_HIP(hipMemcpyAsync(&hrun_sync, &C0->run, sizeof(int), hipMemcpyDeviceToHost, s));
+            CEL_HIP(hipStreamSynchronize(s));
+            if (!hrun_sync) break;
+            ++k;
+            continue;
+        }
         CEL_HIP(hipMemcpyAsync(&hrun[k & 1], &C0->run, sizeof(int), hipMemcpyDeviceToHost, s));
         CEL_HIP(hipEventRecord(cev[k & 1], s));
         if (k > 0) {
@@ -1187,22 +1283,28 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     }
     if (mixed) {
         // a budget that ran out mid-way leaves a correction in e: fold it in
-        hipLaunchKernelGGL(k_ru_flush, dim3(Gv), dim3(kBlock), 0, s, n_ll, dx, W.e32.p, C0, C3, 1);
+        hipLaunchKernelGGL(k_ru_flush, dim3(Gv), dim3(kBlock), 0, s, n_ll, dx, w.e32, C0, C3, 1);
     }
     if (jadp) {  // x back to the original row order
         hipLaunchKernelGGL(k_perm_scatter<double>, dim3(std::max(1, std::min(Gv * 2, 4096))),
                            dim3(kBlock), 0, s, n_ll, A.jad_perm.p, dx, dx_out);
         CEL_HIP(hipGetLastError());
     }
-    CEL_HIP(hipEventRecord(e2, s));
+    if (!noalloc) CEL_HIP(hipEventRecord(e2, s));
     CEL_HIP(hipStreamSynchronize(s));
     if (status < 0) return status;
 
     Ctrl out;
     CEL_HIP(hipMemcpy(&out, C0, sizeof(Ctrl), hipMemcpyDeviceToHost));
     float ms_setup = 0, ms_all = 0;
-    CEL_HIP(hipEventElapsedTime(&ms_setup, e0, e1));
-    CEL_HIP(hipEventElapsedTime(&ms_all, e0, e2));
+    if (!noalloc) {
+        CEL_HIP(hipEventElapsedTime(&ms_setup, e0, e1));
+        CEL_HIP(hipEventElapsedTime(&ms_all, e0, e2));
+    } else {
+        ms_all = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                                  t_host0)
+                     .count();
+    }
     double spmv_ms = 0;
     int spmv_n = 0;
     for (int i = 0; i + 1 < pev_used; i += 2) {
@@ -1217,7 +1319,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         CEL_HIP(hipMemcpy(x, dx_out, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (hist && hist_cap) {
         int cnt = std::min(out.iter + 1, hist_cap);
-        CEL_HIP(hipMemcpy(hist, W.hist.p, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+        CEL_HIP(hipMemcpy(hist, Wp->hist.p, sizeof(double) * cnt, hipMemcpyDeviceToHost));
     }
 
     stats.iters = out.iter;

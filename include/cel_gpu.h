@@ -345,6 +345,22 @@ void cel_gpu_pcg_params_default(cel_pcg_params* p);
 int cel_gpu_pcg_solve(cel_gpu_mat* A, const double* b, double* x, const cel_pcg_params* params,
                       cel_pcg_stats* stats, double* hist);
 
+/* No counterpart in the reference: cel_gpu_pcg_solve for callers that
+ * cannot allocate (code under graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  It calls no hipMalloc, hipFree, hipHostMalloc,
+ * event or graph routine: its work vectors, partial sums and token live in
+ * a fixed device array of the library (about 24.4 MiB per GPU, held until
+ * the process ends), resolved once per device by cel_gpu_init.  Limits:
+ * FP64, CSR, COO or DIA form, no halo blocks, no history, b and x device
+ * pointers (params->memspace must be CEL_MEM_DEVICE), at most 1,048,576
+ * rows (CEL_ERR_ARG beyond) and 8,192 blocks per kernel.  The same bits as
+ * cel_gpu_pcg_solve with use_graph = 0; solve_ms is host time around the
+ * synchronised stream and setup_ms is 0.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (several host threads, or a
+ * stream other than the library's) race. */
+int cel_gpu_pcg_solve_noalloc(cel_gpu_mat* A, const double* b, double* x,
+                              const cel_pcg_params* params, cel_pcg_stats* stats);
+
 /* ---- adjoint (no reference counterpart) -------------------------------- */
 /* For A x = b with L(x): solves A^T lambda = dLdx (A is symmetric here, so
  * the same PCG is used), returns dLdb = lambda and, when dLda is not NULL,
