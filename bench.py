@@ -2,7 +2,7 @@
 """256^3 Jacobi-PCG time-to-1e-10 on one MI355X: mixed precision vs plain FP64,
 and SpMV bandwidth against the HBM roofline.
 
-    python bench.py [--steps K] [--warmup W] [--size N] [--format dia|csr|jad]
+    python bench.py [--steps K] [--warmup W] [--size N] [--format csr|jad|dia]
 
 One step = one full solve of the 7-point Poisson system on an N^3 grid
 (diag 6, off-diagonal -1, Dirichlet outside), b = A x_true with
@@ -10,16 +10,19 @@ x_true ~ U[0,1) (numpy default_rng(1)), x0 = 0, stopped at
 ||b - A x|| <= tol * ||b||.  Inputs are resident in HBM before the timed
 region starts.  Prints one JSON line on stdout; progress goes to stderr.
 
-The timed mixed and FP64 solves use the same local block, stored in the DIA
-form (its products and the whole solve are bit-identical to the CSR form:
-the same row sums in the same order, without column indices); the same
-solves with the CSR form are timed beside them and must give the same bits.
-One more FP64 solve runs the one-subdomain model of the reference's
-distributed SpMV (CSR, which carries the halo blocks): the
-256^3 block carries six halo faces (65,536 entries each) and packs its six
-send faces every SpMV; with no neighbour present the received values are
-zero, so it solves the same Dirichlet problem.  Per-kernel times come from
-HIP events around every K1 launch in separate (untimed) solves.
+`value` is the median mixed-precision solve (FP32 inner iteration, FP64
+residual replacement) in the --format form (CSR by default, the reference's
+format); the plain FP64 solve of the same form is timed beside it.  The
+other two forms of the same block run the same solves (JAD, converted on
+the device; DIA, bitwise the CSR solves), with fewer steps.  Then, untimed:
+per-launch HIP events around the K1 kernel of one solve of each form and
+precision (`roofline*`), the one-subdomain model of the reference's
+distributed SpMV (six halo faces, faces packed every SpMV, received values
+zero) in the CSR and DIA forms, the 384^3 precision workload (x_true ~
+N(0,1), seed 2; mixed vs FP64, CSR), and the CPU side: the oracle's OpenMP
+restatement of the reference loop timed on the host cores (`cpu_baseline`),
+whose 256^3 and 384^3 FP64 solutions are the oracle the GPU solutions are
+compared with.
 """
 from __future__ import annotations
 
@@ -91,13 +94,9 @@ def cpu_baseline(N, tol):
     same 256^3 system (x0 = 0, stop at ||r|| <= tol ||b||, the reference's
     absolute |alpha| test kept).  Returns the timing and the solution, the
     FP64 oracle the GPU solutions are compared with."""
-    # bind the worker threads to cores before the OpenMP runtime starts
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    os.environ.setdefault("OMP_PLACES", "cores")
     import oracle as O
 
-    threads = len(os.sched_getaffinity(0))
-    threads = max(2, min(threads, int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
+    threads = _cpu_threads()
     workers = threads - 1
     cpu_model = "unknown"
     try:
@@ -292,6 +291,82 @@ def sort_cpu_baseline():
             "seconds": dt, "sample": f"160^3 poisson7 COO, {rows.size} entries, shuffled"}
 
 
+def config3(cel, tol, steps, skip_cpu):
+    """The precision workload: 384^3 7-point Poisson (n = 56,623,104, nnz =
+    395,476,992), x_true ~ N(0,1) (numpy default_rng(2)), b = A x_true, x0 =
+    0, to 1e-10: the mixed solve against the plain FP64 solve (CSR form), and
+    both against the oracle's FP64 solve of the same system on the host."""
+    N = 384
+    M = cel.Matrix.poisson7(N, N, N)
+    M.set_jacobi()
+    xt = np.random.default_rng(2).normal(size=M.n)
+    d_xt = cel.DeviceArray.from_host(xt)
+    d_b = cel.DeviceArray(M.n)
+    M.spmv(d_xt, d_b)
+    b = d_b.to_host()
+    bnorm = float(np.linalg.norm(b))
+    kw = dict(tol=tol, tol_mode=cel.TOL_REL_B, x0_mode=cel.X0_ZERO, iter_max=20000,
+              fmt=cel.FMT_CSR, allow_not_converged=False)
+    out = {"workload": f"poisson7_{N}^3_x_true_N(0,1)_seed2_tol{tol:g}", "n": M.n, "nnz": M.nnz,
+           "format": "csr"}
+    runs = {}
+    for name, prec in (("mixed", cel.PREC_MIXED), ("fp64", cel.PREC_FP64)):
+        M.pcg(d_b, precision=prec, **kw)  # warm-up (FP32 values, work vectors)
+        rs = [M.pcg(d_b, precision=prec, **kw) for _ in range(max(1, steps))]
+        runs[name] = rs[-1]
+        out[name + "_solve_ms"] = [r["solve_ms"] for r in rs]
+        log(f"config3 {name}: " + ", ".join(f"{r['solve_ms']:.1f}" for r in rs) + " ms, "
+            f"{rs[-1]['iters']} iterations")
+    xm, xf = runs["mixed"]["x"].to_host(), runs["fp64"]["x"].to_host()
+    res = cel.DeviceArray(M.n)
+    M.spmv(runs["mixed"]["x"], res)
+    out["mixed_inner_iters"] = runs["mixed"]["iters"]
+    out["mixed_fp64_replacements"] = runs["mixed"]["outer_iters"]
+    out["fp64_iters"] = runs["fp64"]["iters"]
+    out["mixed_true_rel_residual"] = float(np.linalg.norm(b - res.to_host()) / bnorm)
+    out["speedup_mixed_vs_fp64"] = float(np.median(out["fp64_solve_ms"]) /
+                                         np.median(out["mixed_solve_ms"]))
+    scale = float(np.abs(xf).max())
+    out["max_rel_diff_mixed_vs_gpu_fp64"] = float(np.abs(xm - xf).max()) / scale
+    out["max_abs_err_vs_x_true"] = {"mixed": float(np.abs(xm - xt).max()),
+                                    "fp64": float(np.abs(xf - xt).max())}
+    M.destroy()
+    del d_xt, d_b, res
+    if not skip_cpu:
+        import oracle as O
+
+        threads = _cpu_threads()
+        t0 = time.time()
+        A = O.gen_poisson7(N, N, N)
+        jac = O.set_jacobi(O.csr_diagonal(A))
+        ro = O.pcg_jacobi(A, jac, b, x0_mode=O.X0_ZERO, eps=tol * bnorm, iter_max=20000,
+                          workers=threads - 1)
+        xo = ro["x"]
+        so = float(np.abs(xo).max())
+        out["oracle"] = {"iters": ro["iters"], "seconds": time.time() - t0,
+                         "max_abs_err_vs_x_true": float(np.abs(xo - xt).max())}
+        for name, x in (("mixed", xm), ("fp64", xf)):
+            out[f"{name}_max_rel_diff_vs_fp64_oracle"] = float(np.abs(x - xo).max()) / so
+        log(f"config3 oracle: {ro['iters']} iterations in {time.time() - t0:.1f} s; "
+            f"mixed vs oracle {out['mixed_max_rel_diff_vs_fp64_oracle']:.3e}, "
+            f"fp64 vs oracle {out['fp64_max_rel_diff_vs_fp64_oracle']:.3e}")
+        del A, jac
+    return out
+
+
+_THREADS = []
+
+
+def _cpu_threads():
+    """Host threads for the oracle: the cores this process may run on (at
+    most OMP_NUM_THREADS), counted once, before an OpenMP runtime binds the
+    main thread to one core."""
+    if not _THREADS:
+        threads = len(os.sched_getaffinity(0))
+        _THREADS.append(max(2, min(threads, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))))
+    return _THREADS[0]
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -299,18 +374,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--format", default="dia", choices=["dia", "csr", "jad"])
+    ap.add_argument("--format", default="csr", choices=["csr", "jad", "dia"],
+                    help="form of the block for `value` (the others are timed beside it)")
+    ap.add_argument("--side-steps", type=int, default=5,
+                    help="timed solves of each of the other two forms (at most --steps)")
     ap.add_argument("--skip-cpu", action="store_true", help="do not time the CPU baseline")
+    ap.add_argument("--skip-config3", action="store_true", help="do not run the 384^3 workload")
     ap.add_argument("--only", default="all",
-                    choices=["all", "fp64", "mixed", "spmv", "profile", "batch", "sort", "gmres"],
+                    choices=["all", "fp64", "mixed", "spmv", "profile", "batch", "sort", "gmres",
+                             "config3"],
                     help="run one part only; 'profile' = one streaming copy of known size "
                          "(counter calibration) and FP64, FP64-with-halo and mixed solves "
-                         "capped at --iters iterations, for hardware-counter runs; 'batch' = "
-                         "the batched workload (64 variable-coefficient systems + 128^3 with 16 "
-                         "right-hand sides) against one-by-one solves; 'sort' = COO ordering of "
-                         "the shuffled --size^3 matrix in HBM; 'gmres' = Jacobi-GMRES(3) with "
-                         "the element solver's settings on the 128^3 block (--iters > 0 caps "
-                         "its outer iterations)")
+                         "capped at --iters iterations in every form, for hardware-counter "
+                         "runs; 'batch' = the batched workload (64 variable-coefficient systems "
+                         "+ 128^3 with 16 right-hand sides) against one-by-one solves; 'sort' = "
+                         "COO ordering of the shuffled --size^3 matrix in HBM; 'gmres' = "
+                         "Jacobi-GMRES(3) with the element solver's settings on the 128^3 block "
+                         "(--iters > 0 caps its outer iterations); 'config3' = the 384^3 "
+                         "precision workload")
     ap.add_argument("--iters", type=int, default=None,
                     help="iteration cap of --only profile (default 20) and --only gmres (default none)")
     args = ap.parse_args()
@@ -320,6 +401,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    # bind the oracle's OpenMP workers to cores before its runtime starts
+    _cpu_threads()
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import cel_v3_amd as cel
 
     if args.only == "batch":
@@ -334,6 +419,10 @@ def main():
         cel.init(0)
         print(json.dumps(sort_bench(cel, args.size, args.steps, args.warmup, args.skip_cpu)))
         return
+    if args.only == "config3":
+        cel.init(0)
+        print(json.dumps(config3(cel, args.tol, args.steps, args.skip_cpu)))
+        return
 
     N = args.size
     cel.init(0)
@@ -341,20 +430,17 @@ def main():
     log(f"device {dev}, {cus} CUs; building {N}^3 Poisson on the device")
     M = cel.Matrix.poisson7(N, N, N)
     M.set_jacobi()
-    fmt = cel.FMT_CSR
-    if args.format == "jad":
-        M.to_jad()
-        fmt = cel.FMT_JAD
-    if args.format == "dia":
-        t0 = time.perf_counter()
-        offs = M.to_dia()
-        log(f"DIA form: {len(offs)} diagonals {offs}, built in {(time.perf_counter() - t0) * 1e3:.1f} ms")
-        fmt = cel.FMT_DIA
-    use_halo = int(args.format in ("csr", "dia"))
-    if use_halo:
-        for proc, rows, cols in face_blocks(N):
-            M.add_halo_block(proc, proc * N ** 3 + 1, N ** 3, rows, cols, -np.ones(rows.size))
-            M.set_send_list(proc, rows)
+    FORMS = {"csr": cel.FMT_CSR, "jad": cel.FMT_JAD, "dia": cel.FMT_DIA}
+    t0 = time.perf_counter()
+    offs = M.to_dia()
+    log(f"DIA form: {len(offs)} diagonals {offs}, built in {(time.perf_counter() - t0) * 1e3:.1f} ms")
+    t0 = time.perf_counter()
+    M.to_jad()
+    jad_convert_ms = (time.perf_counter() - t0) * 1e3
+    log(f"JAD form: converted on the device in {jad_convert_ms:.1f} ms")
+    head = args.format
+    fmt = FORMS[head]
+    sides = [f for f in ("csr", "jad", "dia") if f != head]
     xt = np.random.default_rng(1).random(M.n)
     d_xt = cel.DeviceArray.from_host(xt)
     d_b = cel.DeviceArray(M.n)
@@ -362,11 +448,10 @@ def main():
     bnorm = float(np.linalg.norm(d_b.to_host()))
     log(f"n={M.n} nnz={M.nnz} |b|={bnorm:.6e}")
 
-    def solve(precision, profile=0, halo=0, form=None):
-        f = fmt if form is None else form
-        r = M.pcg(d_b, x0=None, tol=args.tol, tol_mode=cel.TOL_REL_B, x0_mode=cel.X0_ZERO,
-                  iter_max=20000, precision=precision, fmt=f, profile=profile, use_halo=halo,
-                  allow_not_converged=False)
+    def solve(precision, form, profile=0, halo=0, mat=None):
+        r = (mat or M).pcg(d_b, x0=None, tol=args.tol, tol_mode=cel.TOL_REL_B, x0_mode=cel.X0_ZERO,
+                           iter_max=20000, precision=precision, fmt=FORMS[form], profile=profile,
+                           use_halo=halo, allow_not_converged=False)
         return r
 
     solutions = {}
@@ -385,20 +470,31 @@ def main():
     if args.only == "profile":
         copy_bytes = 256 * 1024 ** 2
         ms = cel.stream_copy_time(copy_bytes, reps=2)
-        prof = {"copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes,
-                "copy_ms": ms}
-        runs = [("fp64", cel.PREC_FP64, 0, fmt), ("fp64_halo", cel.PREC_FP64, use_halo, fmt),
-                ("mixed", cel.PREC_MIXED, 0, fmt)]
-        if fmt == cel.FMT_DIA:
-            runs += [("fp64_csr", cel.PREC_FP64, 0, cel.FMT_CSR), ("mixed_csr", cel.PREC_MIXED, 0, cel.FMT_CSR),
-                     ("fp64_halo_csr", cel.PREC_FP64, use_halo, cel.FMT_CSR)]
-        for name, prec, halo, f in runs:
-            r = M.pcg(d_b, x0=None, tol=args.tol, tol_mode=cel.TOL_REL_B, x0_mode=cel.X0_ZERO,
-                      iter_max=args.iters or 20, precision=prec, fmt=f, use_halo=halo, profile=1)
-            prof[name] = {"iters": r["iters"], "spmv_bytes": r["spmv_bytes"],
-                          "mean_spmv_ms": r["spmv_ms"] / max(r["spmv_launches"], 1)}
-            log(f"profile {name}: {r['iters']} iterations, K1 "
-                f"{prof[name]['mean_spmv_ms'] * 1e3:.1f} us")
+        prof = {"copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes, "copy_ms": ms}
+        H = None
+        for form in ("csr", "jad", "dia"):
+            runs = [("fp64_" + form, cel.PREC_FP64, 0), ("mixed_" + form, cel.PREC_MIXED, 0)]
+            if form != "jad":
+                runs.append(("fp64_halo_" + form, cel.PREC_FP64, 1))
+            for name, prec, halo in runs:
+                mat = M
+                if halo:
+                    if H is None:
+                        H = cel.Matrix.poisson7(N, N, N)
+                        H.set_jacobi()
+                        for proc, rows, cols in face_blocks(N):
+                            H.add_halo_block(proc, proc * N ** 3 + 1, N ** 3, rows, cols,
+                                             -np.ones(rows.size))
+                            H.set_send_list(proc, rows)
+                        H.to_dia()
+                    mat = H
+                r = mat.pcg(d_b, x0=None, tol=args.tol, tol_mode=cel.TOL_REL_B, x0_mode=cel.X0_ZERO,
+                            iter_max=args.iters or 20, precision=prec, fmt=FORMS[form], use_halo=halo,
+                            profile=1)
+                prof[name] = {"iters": r["iters"], "spmv_bytes": r["spmv_bytes"],
+                              "mean_spmv_ms": r["spmv_ms"] / max(r["spmv_launches"], 1)}
+                log(f"profile {name}: {r['iters']} iterations, K1 "
+                    f"{prof[name]['mean_spmv_ms'] * 1e3:.1f} us")
         print(json.dumps({"profile": prof, "n": M.n, "nnz": M.nnz}))
         return
     # ------------------------------------------------------------ SpMV kernels alone
@@ -408,26 +504,13 @@ def main():
         y64 = cel.DeviceArray(M.n)
         x32 = cel.DeviceArray.from_host(xt.astype(np.float32))
         y32 = cel.DeviceArray(M.n, np.float32)
-        spmv = {}
-        forms = [("csr_fp64", cel.FMT_CSR, cel.PREC_FP64, x64, y64),
-                 ("csr_fp32", cel.FMT_CSR, cel.PREC_MIXED, x32, y32)]
-        if fmt == cel.FMT_DIA:
-            forms += [("dia_fp64", cel.FMT_DIA, cel.PREC_FP64, x64, y64),
-                      ("dia_fp32", cel.FMT_DIA, cel.PREC_MIXED, x32, y32)]
-        for name, f, prec, x, y in forms:
-            ms, nb = M.spmv_time(x, y, fmt=f, precision=prec, reps=20)
-            spmv[name] = {"ms": ms, "bytes": nb, "GBs": nb / ms * 1e-6,
-                          "frac_hbm": nb / ms * 1e-6 / HBM_PEAK_GBS}
-        # the JAD form of the same block, converted on the device
-        if fmt != cel.FMT_JAD:
-            t0 = time.perf_counter()
-            M.to_jad()
-            spmv["jad_convert_ms"] = (time.perf_counter() - t0) * 1e3
-        for name, prec, x, y in (("jad_fp64", cel.PREC_FP64, x64, y64),
-                                 ("jad_fp32", cel.PREC_MIXED, x32, y32)):
-            ms, nb = M.spmv_time(x, y, fmt=cel.FMT_JAD, precision=prec, reps=20)
-            spmv[name] = {"ms": ms, "bytes": nb, "GBs": nb / ms * 1e-6,
-                          "frac_hbm": nb / ms * 1e-6 / HBM_PEAK_GBS}
+        spmv = {"jad_convert_ms": jad_convert_ms}
+        for form in ("csr", "jad", "dia"):
+            for pname, prec, x, y in (("fp64", cel.PREC_FP64, x64, y64),
+                                      ("fp32", cel.PREC_MIXED, x32, y32)):
+                ms, nb = M.spmv_time(x, y, fmt=FORMS[form], precision=prec, reps=20)
+                spmv[f"{form}_{pname}"] = {"ms": ms, "bytes": nb, "GBs": nb / ms * 1e-6,
+                                           "frac_hbm": nb / ms * 1e-6 / HBM_PEAK_GBS}
         cb = 2 * 1024 ** 3
         ms = cel.stream_copy_time(cb, reps=10)
         spmv["stream_copy_GBs"] = 2 * cb / ms * 1e-6
@@ -444,61 +527,69 @@ def main():
     for w in range(args.warmup):
         log(f"warm-up {w + 1}/{args.warmup}")
         if args.only in ("all", "fp64"):
-            solve(cel.PREC_FP64)
+            solve(cel.PREC_FP64, head)
         if args.only in ("all", "mixed"):
-            solve(cel.PREC_MIXED)
+            solve(cel.PREC_MIXED, head)
 
     # ------------------------------------------------------------ timed region
-    # both forms solve the same local 256^3 block without halo blocks, so
-    # the mixed/FP64 ratio compares like with like; neither is profiled
+    # both precisions solve the same local 256^3 block without halo blocks,
+    # so the mixed/FP64 ratio compares like with like; neither is profiled
     fp64_runs, mixed_runs = [], []
     cel.lib().cel_gpu_synchronize()
     t_start = time.perf_counter()
     for k in range(args.steps):
         if args.only in ("all", "mixed"):
-            r = solve(cel.PREC_MIXED)
+            r = solve(cel.PREC_MIXED, head)
             mixed_runs.append(r)
-            log(f"step {k + 1}: mixed {r['solve_ms']:.1f} ms, {r['iters']} inner iterations, "
+            log(f"step {k + 1}: mixed {head} {r['solve_ms']:.1f} ms, {r['iters']} inner iterations, "
                 f"{r['outer_iters']} FP64 replacements")
     cel.lib().cel_gpu_synchronize()
     t_mixed = time.perf_counter() - t_start
     t_start = time.perf_counter()
     for k in range(args.steps):
         if args.only in ("all", "fp64"):
-            r = solve(cel.PREC_FP64)
+            r = solve(cel.PREC_FP64, head)
             fp64_runs.append(r)
-            log(f"step {k + 1}: fp64 {r['solve_ms']:.1f} ms, {r['iters']} iterations")
+            log(f"step {k + 1}: fp64 {head} {r['solve_ms']:.1f} ms, {r['iters']} iterations")
     cel.lib().cel_gpu_synchronize()
     t_fp64 = time.perf_counter() - t_start
-    # the same solves with the CSR form (like-for-like with the reference's
-    # format); their solutions must equal the DIA ones bit for bit
-    csr_runs = {}
-    if fmt == cel.FMT_DIA and args.only == "all":
-        for name, prec in (("mixed", cel.PREC_MIXED), ("fp64", cel.PREC_FP64)):
-            solve(prec, form=cel.FMT_CSR)  # warm-up
-            rs = [solve(prec, form=cel.FMT_CSR) for _ in range(args.steps)]
-            csr_runs[name] = rs
-            log(f"csr {name}: " + ", ".join(f"{r['solve_ms']:.1f}" for r in rs) + " ms")
+
+    # the same solves in the other two forms
+    side = {}
+    if args.only == "all":
+        ns = max(1, min(args.side_steps, args.steps))
+        for form in sides:
+            side[form] = {}
+            for name, prec in (("mixed", cel.PREC_MIXED), ("fp64", cel.PREC_FP64)):
+                solve(prec, form)  # warm-up
+                rs = [solve(prec, form) for _ in range(ns)]
+                side[form][name] = rs
+                log(f"{form} {name}: " + ", ".join(f"{r['solve_ms']:.1f}" for r in rs) + " ms")
 
     # ------------------------------------------------------------ per-kernel timing (untimed)
     # one more solve of each form with HIP events around every K1 launch on
     # the library stream, and the one-subdomain halo model of the FP64 form
     prof = {}
-    if args.only in ("all", "mixed"):
-        prof["mixed"] = solve(cel.PREC_MIXED, profile=1)
-        if fmt == cel.FMT_DIA:
-            prof["mixed_csr"] = solve(cel.PREC_MIXED, profile=1, form=cel.FMT_CSR)
-    if args.only in ("all", "fp64"):
-        prof["fp64"] = solve(cel.PREC_FP64, profile=1)
-        if fmt == cel.FMT_DIA:
-            prof["fp64_csr"] = solve(cel.PREC_FP64, profile=1, form=cel.FMT_CSR)
-        if use_halo:
-            halo_timed = {"dia" if fmt == cel.FMT_DIA else "csr": solve(cel.PREC_FP64, halo=1)}
-            prof["fp64_halo"] = solve(cel.PREC_FP64, profile=1, halo=1)
-            if fmt == cel.FMT_DIA:
-                halo_timed["csr"] = solve(cel.PREC_FP64, halo=1, form=cel.FMT_CSR)
-                prof["fp64_halo_csr"] = solve(cel.PREC_FP64, profile=1, halo=1, form=cel.FMT_CSR)
-            log("fp64 with halo faces: " + ", ".join(f"{k} {v['solve_ms']:.1f} ms" for k, v in halo_timed.items()))
+    halo_timed = {}
+    if args.only == "all":
+        for form in ("csr", "jad", "dia"):
+            prof["mixed_" + form] = solve(cel.PREC_MIXED, form, profile=1)
+            prof["fp64_" + form] = solve(cel.PREC_FP64, form, profile=1)
+        H = cel.Matrix.poisson7(N, N, N)
+        H.set_jacobi()
+        for proc, rows, cols in face_blocks(N):
+            H.add_halo_block(proc, proc * N ** 3 + 1, N ** 3, rows, cols, -np.ones(rows.size))
+            H.set_send_list(proc, rows)
+        H.to_dia()
+        # the local solve of the same handle (its launch shapes) to compare with
+        halo_local = solve(cel.PREC_FP64, "csr", mat=H)["x"].to_host()
+        for form in ("csr", "dia"):
+            solve(cel.PREC_FP64, form, halo=1, mat=H)  # warm-up
+            halo_timed[form] = solve(cel.PREC_FP64, form, halo=1, mat=H)
+            prof["fp64_halo_" + form] = solve(cel.PREC_FP64, form, profile=1, halo=1, mat=H)
+        log("fp64 with halo faces: " + ", ".join(f"{k} {v['solve_ms']:.1f} ms" for k, v in halo_timed.items()))
+        halo_x = {k: v["x"].to_host() for k, v in halo_timed.items()}
+        H.destroy()
 
     def k1(r):
         ms = r["spmv_ms"] / max(r["spmv_launches"], 1)
@@ -508,7 +599,7 @@ def main():
                higher_is_better=False, scaling="weak", vs_baseline=None, data="synthetic",
                unit="s")
     cfg = {"workload": f"poisson7_{N}^3_jacobi_pcg_tol{args.tol:g}", "n": M.n, "nnz": M.nnz,
-           "format": args.format, "x_true": "U[0,1) numpy default_rng(1)", "x0": 0,
+           "format": head, "x_true": "U[0,1) numpy default_rng(1)", "x0": 0,
            "stop": "||b-Ax|| <= tol*||b||", "parallelism": "1 GPU, one subdomain"}
     if mixed_runs:
         ms = [r["solve_ms"] for r in mixed_runs]
@@ -516,21 +607,21 @@ def main():
         res["value"] = float(np.median(ms)) / 1e3
         res["ms_per_step"] = t_mixed / args.steps * 1e3
         res["dtype"] = "f32+f64"
-        res["mixed"] = {"solve_ms": ms, "inner_iters": mixed_runs[-1]["iters"],
+        res["mixed"] = {"format": head, "solve_ms": ms, "inner_iters": mixed_runs[-1]["iters"],
                         "fp64_replacements": mixed_runs[-1]["outer_iters"],
                         "true_rel_residual": rel, "max_abs_err_vs_x_true": err,
                         "wall_ms_per_step": t_mixed / args.steps * 1e3,
                         "iter_bytes": mixed_runs[-1]["iter_bytes"]}
-        log(f"mixed: {res['value']:.4f} s, true rel residual {rel:.3e}, max err {err:.3e}")
+        log(f"mixed {head}: {res['value']:.4f} s, true rel residual {rel:.3e}, max err {err:.3e}")
     if fp64_runs:
         ms = [r["solve_ms"] for r in fp64_runs]
         rel, err = check(fp64_runs[-1], "fp64")
         last = fp64_runs[-1]
-        res["fp64"] = {"solve_ms": ms, "iters": last["iters"], "true_rel_residual": rel,
+        res["fp64"] = {"format": head, "solve_ms": ms, "iters": last["iters"], "true_rel_residual": rel,
                        "max_abs_err_vs_x_true": err, "wall_ms_per_step": t_fp64 / args.steps * 1e3,
                        "iter_bytes": last["iter_bytes"],
                        "iter_GBs": last["iter_bytes"] * last["iters"] / (np.median(ms) * 1e-3) / 1e9}
-        log(f"fp64: {np.median(ms) / 1e3:.4f} s, {last['iters']} iterations")
+        log(f"fp64 {head}: {np.median(ms) / 1e3:.4f} s, {last['iters']} iterations")
         if "value" not in res:
             res["value"] = float(np.median(ms)) / 1e3
             res["ms_per_step"] = t_fp64 / args.steps * 1e3
@@ -538,34 +629,35 @@ def main():
     if mixed_runs and fp64_runs:
         res["speedup_mixed_vs_fp64"] = (np.median([r["solve_ms"] for r in fp64_runs]) /
                                         np.median([r["solve_ms"] for r in mixed_runs]))
-    if csr_runs:
-        same = {}
-        for name, runs_ in (("mixed", mixed_runs), ("fp64", fp64_runs)):
-            a, b = runs_[-1]["x"].to_host(), csr_runs[name][-1]["x"].to_host()
-            same[name] = bool(np.array_equal(a, b))
-        res["csr"] = {"note": "the same solves with the CSR form",
-                      "mixed_solve_ms": [r["solve_ms"] for r in csr_runs["mixed"]],
-                      "fp64_solve_ms": [r["solve_ms"] for r in csr_runs["fp64"]],
-                      "solutions_bitwise_equal_to_dia": same}
-        res["dia_speedup_vs_csr"] = {
-            k: float(np.median(res["csr"][k + "_solve_ms"]) /
-                     np.median([r["solve_ms"] for r in (mixed_runs if k == "mixed" else fp64_runs)]))
-            for k in ("mixed", "fp64")}
-        log(f"DIA vs CSR: {res['dia_speedup_vs_csr']}, bitwise equal: {same}")
-    # the dominant kernel of the timed (mixed) solve is its FP32 K1; the FP64
-    # K1 is reported beside it
-    fname = {cel.FMT_DIA: "DIA", cel.FMT_CSR: "CSR", cel.FMT_JAD: "JAD"}[fmt]
-    for key, form, name in (("roofline", "mixed", f"k_pcg_spmv<float,{fname}> (v = A p fused with v.p)"),
-                            ("roofline_fp64", "fp64", f"k_pcg_spmv<double,{fname}> (v = A p fused with v.p)"),
-                            ("roofline_csr", "mixed_csr", "k_pcg_spmv<float,CSR> (v = A p fused with v.p)"),
-                            ("roofline_csr_fp64", "fp64_csr", "k_pcg_spmv<double,CSR> (v = A p fused with v.p)"),
-                            ("roofline_fp64_halo", "fp64_halo",
-                             f"k_pcg_spmv<double,{fname},halo> (+6 halo faces, one-subdomain model)"),
-                            ("roofline_csr_fp64_halo", "fp64_halo_csr",
-                             "k_pcg_spmv<double,CSR,halo> (+6 halo faces, one-subdomain model)")):
-        if form not in prof:
+    for form, rs in side.items():
+        entry = {"mixed_solve_ms": [r["solve_ms"] for r in rs["mixed"]],
+                 "fp64_solve_ms": [r["solve_ms"] for r in rs["fp64"]],
+                 "mixed_inner_iters": rs["mixed"][-1]["iters"], "fp64_iters": rs["fp64"][-1]["iters"]}
+        entry["speedup_mixed_vs_fp64"] = float(np.median(entry["fp64_solve_ms"]) /
+                                               np.median(entry["mixed_solve_ms"]))
+        for name in ("mixed", "fp64"):
+            solutions[f"{form}_{name}"] = rs[name][-1]["x"].to_host()
+        if "csr" in (head, form) and "dia" in (head, form):
+            # the DIA form's products are the CSR ones bit for bit
+            entry["solutions_bitwise_equal_to_csr" if form == "dia" else "solutions_bitwise_equal_to_dia"] = {
+                name: bool(np.array_equal(solutions[f"{form}_{name}"], solutions[name]))
+                for name in ("mixed", "fp64") if name in solutions}
+        res[form] = entry
+    fname = {"csr": "CSR", "jad": "JAD", "dia": "DIA"}
+    roof_keys = [("roofline", "mixed_" + head, "float", head),
+                 ("roofline_fp64", "fp64_" + head, "double", head)]
+    for form in sides:
+        roof_keys += [(f"roofline_{form}", "mixed_" + form, "float", form),
+                      (f"roofline_{form}_fp64", "fp64_" + form, "double", form)]
+    roof_keys += [("roofline_fp64_halo", "fp64_halo_" + "csr", "double", "csr"),
+                  ("roofline_dia_fp64_halo", "fp64_halo_" + "dia", "double", "dia")]
+    for key, pname, vt, form in roof_keys:
+        if pname not in prof:
             continue
-        r = prof[form]
+        r = prof[pname]
+        halo = "halo" in pname
+        name = (f"k_pcg_spmv<{vt},{fname[form]}" + (",halo>" if halo else ">") +
+                (" (+6 halo faces, one-subdomain model)" if halo else " (v = A p fused with v.p)"))
         k1_ms, achieved = k1(r)
         res[key] = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -574,31 +666,36 @@ def main():
                     "launches": r["spmv_launches"], "solve_ms_profiled": r["solve_ms"]}
         log(f"{key}: {name.split(' ')[0]} {k1_ms * 1e3:.1f} us = {achieved:.0f} GB/s "
             f"({achieved / HBM_PEAK_GBS:.1%} of 8 TB/s)")
-    if "fp64_halo" in prof:
-        r = prof["fp64_halo"]
-        first = next(iter(halo_timed.values()))
-        res["fp64_halo"] = {"solve_ms": first["solve_ms"], "iters": first["iters"], "format": args.format,
-                            "halo_faces": 6, "face_entries": 6 * N * N,
+    if halo_timed:
+        res["fp64_halo"] = {"csr_solve_ms": halo_timed["csr"]["solve_ms"],
+                            "dia_solve_ms": halo_timed["dia"]["solve_ms"],
+                            "iters": halo_timed["csr"]["iters"], "halo_faces": 6,
+                            "face_entries": 6 * N * N,
                             "note": "one-subdomain model: 6 halo blocks and face packing every "
-                                    "SpMV, received values zero"}
-        if "csr" in halo_timed and fmt == cel.FMT_DIA:
-            rc = halo_timed["csr"]
-            res["fp64_halo"]["csr_solve_ms"] = rc["solve_ms"]
-            res["fp64_halo"]["solution_bitwise_equal_to_csr"] = bool(
-                np.array_equal(first["x"].to_host(), rc["x"].to_host()))
+                                    "SpMV, received values zero",
+                            "solutions_bitwise_equal_to_local_csr_solve": {
+                                k: bool(np.array_equal(v, halo_local)) for k, v in halo_x.items()}}
     res["config"] = cfg
     res.update(out)
+    if not args.skip_config3 and args.only == "all":
+        res["config3"] = config3(cel, args.tol, 1, args.skip_cpu)
     if not args.skip_cpu and fp64_runs and args.only == "all":
         base, x_oracle = cpu_baseline(N, args.tol)
         res["cpu_baseline"] = base
         # error of each GPU solution against the FP64 oracle's solution
         scale = float(np.abs(x_oracle).max())
-        for name in ("fp64", "mixed"):
-            if name in solutions and name in res:
-                d = float(np.abs(solutions[name] - x_oracle).max())
-                res[name]["max_abs_diff_vs_fp64_oracle"] = d
-                res[name]["max_rel_diff_vs_fp64_oracle"] = d / scale
-                log(f"{name}: max |x - x_oracle| = {d:.3e} ({d / scale:.3e} of max |x|)")
+        eo = float(np.abs(x_oracle - xt).max())
+        res["fp64_oracle"] = {"iters": base["iters"], "max_abs_err_vs_x_true": eo}
+        for key, sol in (("mixed", "mixed"), ("fp64", "fp64")):
+            if sol in solutions and key in res:
+                d = float(np.abs(solutions[sol] - x_oracle).max())
+                res[key]["max_abs_diff_vs_fp64_oracle"] = d
+                res[key]["max_rel_diff_vs_fp64_oracle"] = d / scale
+                log(f"{key}: max |x - x_oracle| = {d:.3e} ({d / scale:.3e} of max |x|)")
+        for form in side:
+            for name in ("mixed", "fp64"):
+                d = float(np.abs(solutions[f"{form}_{name}"] - x_oracle).max())
+                res[form][f"{name}_max_rel_diff_vs_fp64_oracle"] = d / scale
         res["fp64"]["oracle_iters"] = base["iters"]
     print(json.dumps(res))
 

@@ -925,7 +925,9 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
                               (const void*)k_residual<float, CEL_FMT_JAD, false, true>}, ntiles);
         }
     }
-    {
+    if (A.halo.empty()) {
+        // (a block with halo blocks keeps every resident block: its K1s wait
+        // on the halo products' loads, which more blocks hide)
         const int cap = rt().num_cus * (mixed ? kK1BlocksPerCuF32 : kK1BlocksPerCuF64);
         if (G1 > cap) G1 = cap >= kXcds ? cap / kXcds * kXcds : cap;
     }

@@ -14,7 +14,7 @@ Bars:
   * the DIA form: bitwise the CSR solve (FP64 and mixed);
   * the one-subdomain halo model (six halo faces of 65,536 entries, the six
     send faces packed every SpMV, received values zero): bitwise the local
-    solve, CSR and DIA;
+    solve of the same handle, CSR and DIA, which matches the oracle;
   * mixed precision (FP32 inner iteration, FP64 residual replacement), CSR
     and JAD: FP64 true residual <= 1e-10 ||b||, and the solution no further
     from x_true, nor from the oracle's FP64 solution, than twice the
@@ -129,18 +129,23 @@ def test_dia_and_halo_forms_are_bitwise_the_csr_solve(gpu, c2):
     x = csr["x"].to_host()
     assert csr["iters"] == dia["iters"] == c2["ref"]["iters"]
     np.testing.assert_array_equal(dia["x"].to_host(), x)
-    # one subdomain with its six halo faces; received values are zero
+    # one subdomain with its six halo faces; received values are zero, so
+    # the halo solve is the local solve of the same handle (same launch
+    # shapes) bit for bit, and that one matches the oracle
     H = gpu.Matrix.poisson7(N, N, N)
     H.set_jacobi()
     _face_blocks(H)
     assert H.ghost_len() == (6 * N * N, 6 * N * N)
     H.to_dia()
+    kw = dict(tol=TOL, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, iter_max=20000,
+              allow_not_converged=False)
+    loc = H.pcg(c2["d_b"], fmt=gpu.FMT_CSR, history=True, **kw)
+    xl = _check_fp64(c2, loc, "local solve of the handle with halo blocks")
     for f, name in ((gpu.FMT_CSR, "csr"), (gpu.FMT_DIA, "dia")):
-        h = H.pcg(c2["d_b"], tol=TOL, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, iter_max=20000,
-                  fmt=f, use_halo=1, allow_not_converged=False)
+        h = H.pcg(c2["d_b"], fmt=f, use_halo=1, **kw)
         print(f"halo {name}: {h['iters']} iterations, {h['solve_ms']:.1f} ms", flush=True)
-        assert h["iters"] == csr["iters"]
-        np.testing.assert_array_equal(h["x"].to_host(), x)
+        assert h["iters"] == loc["iters"]
+        np.testing.assert_array_equal(h["x"].to_host(), xl)
     H.destroy()
     # mixed: the DIA form is bitwise the CSR form
     mc = _solve(gpu, c2, gpu.FMT_CSR, gpu.PREC_MIXED)
