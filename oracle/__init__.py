@@ -17,7 +17,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liboracle.so")
+# CEL_ORACLE_LIB names another build of the same source (the sanitizer run)
+_LIB = os.environ.get("CEL_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _REF = os.path.join(_HERE, "_ref", "libcel_base_ref.so")
 _REF_MM = os.path.join(_HERE, "_ref", "libcel_mm_ref.so")
 

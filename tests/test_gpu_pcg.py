@@ -270,7 +270,7 @@ def test_config3_mixed_384(gpu):
           f"{true_rel:.3e}, max|x_mixed - x_fp64|/max|x| {d_fp64:.3e}, max err vs x_true {err:.3e}",
           flush=True)
     assert true_rel <= 1e-10
-    assert d_fp64 <= 1e-7
+    assert d_fp64 <= 1e-8  # measured 3.4e-11 (seed 2)
     assert abs(mixed["iters"] - fp64["iters"]) <= 0.05 * fp64["iters"]
 
 

@@ -8,6 +8,8 @@ and independent library routines (numpy dense algebra, scipy's CG).
 """
 import os
 
+from fractions import Fraction
+
 import numpy as np
 import pytest
 
@@ -783,6 +785,85 @@ def test_gmres_final_norm_is_the_true_residual_and_shows_the_rotation_defect():
             assert ref["norm"] <= 1.01e-8
         else:
             assert ref["norm"] > 100 * 1e-8
+
+
+# The hand-trace system: A = 2 V H V^T with V = [[1,2,2],[2,1,-2],[2,-2,1]]/3
+# (orthonormal columns v1, v2, v3) and H = [[3,1,0],[4,5,-13],[0,12,-5]]
+# (upper Hessenberg), so diag(A) = 2, jac = 1/diag(A) = 1/2 and A jac = V H V^T:
+# the Arnoldi vectors are v1, v2, v3 and the Arnoldi columns are those of H.
+# x0 = 1 and b = A 1 + (1, 2, 2), so r0 = 3 v1.
+GM3_A = [[2.0, 86 / 3, -56 / 3], [-20 / 3, 2.0, 34 / 3], [32 / 3, -28 / 3, 2.0]]
+GM3_B = [13.0, 26 / 3, 16 / 3]
+
+
+def gm3_system():
+    A = O.CSR(np.array([1, 4, 7, 10], np.int32), np.tile(np.arange(1, 4, dtype=np.int32), 3),
+              np.array(GM3_A).ravel())
+    return A, np.full(3, 0.5), np.array(GM3_B)
+
+
+def gm3_x(alphas):
+    """x0 + sum_i alpha_i z_i with z_i = jac v_i = v_i / 2, in exact arithmetic."""
+    v = [(1, 2, 2), (2, 1, -2), (2, -2, 1)]
+    return [1 + sum(a * Fraction(vi[r], 6) for a, vi in zip(alphas, v)) for r in range(3)]
+
+
+def test_gmres_hand_trace_restart3_converges_with_the_reference_rotation():
+    """cel_gmresalg_2jacobi (src/fcgalg/cel_gmresalg_module.f90:637-851) by
+    hand on the 3x3 system above, m = 3, in exact arithmetic.
+      r0 = (1,2,2): |r0| = 3, gamma = (3).
+      jj=1: w = A z1 = 3 v1 + 4 v2; h11 = 3, h21 = 4; beta = 5, c1 = 3/5,
+            s1 = 4/5, h11 <- 5; gamma = (9/5, -12/5).
+      jj=2: w = A z2 = v1 + 5 v2 + 12 v3; h12 = 1, h22 = 5, h32 = 12.
+            Rotation 1 as the reference writes it (:722-727; both statements
+            assign h(1,2), h(2,2) is never rotated): t = 1*3/5 + 5*4/5 =
+            23/5, h12 <- -t*4/5 + 5*3/5 = -17/25.  beta = sqrt(5^2 + 12^2)
+            = 13, c2 = 5/13, s2 = 12/13, h22 <- 13; gamma = (9/5, -12/13,
+            144/65).
+      jj=3: w = A z3 = -13 v2 - 5 v3; h13 = 0, h23 = -13, h33 = -5, h43 = 0.
+            Rotation 1: t = 0*3/5 - 13*4/5 = -52/5, h13 <- 52/5*4/5 - 13*3/5
+            = 13/25.  Rotation 2 (reads the unrotated h23 = -13): t = -13*5/13
+            - 5*12/13 = -125/13, h23 <- 125/13*12/13 - 5*5/13 = 1175/169.
+            beta = |h33| = 5, c3 = -1, s3 = 0, h33 <- 5; gamma4 = 0 <= eps,
+            gamma3 <- -144/65.
+      back substitution: alpha3 = (-144/65)/5 = -144/325, alpha2 = (-12/13 -
+            1175/169*alpha3)/13 = 4740/28561, alpha1 = (9/5 + 17/25*alpha2 -
+            13/25*alpha3)/5 = 38258409/89253125.
+    One cycle, reason XN.  A textbook rotation would solve this 3x3 system
+    exactly in 3 steps; the reference's lands at a residual of 9.437."""
+    A, jac, b = gm3_system()
+    x = gm3_x([Fraction(38258409, 89253125), Fraction(4740, 28561), Fraction(-144, 325)])
+    assert x == [Fraction(174770053, 178506250), Fraction(117656678, 89253125),
+                 Fraction(90477428, 89253125)]
+    ref = O.gmres_jacobi(A, jac, b, eps=1e-10, m=3, iter_max=50)
+    assert (ref["iters"], ref["cycles"], ref["reason"]) == (1, 1, 2)
+    np.testing.assert_allclose(ref["x"], [float(v) for v in x], rtol=0, atol=1e-14)
+    r = [Fraction(GM3_B[i]) - sum(Fraction(GM3_A[i][j]) * x[j] for j in range(3)) for i in range(3)]
+    true = float(sum(v * v for v in r)) ** 0.5
+    assert abs(ref["norm"] - true) <= 1e-13 * true and 9.43 < true < 9.44
+    assert np.abs(ref["x"] - np.linalg.solve(np.array(GM3_A), b)).max() > 0.1
+
+
+def test_gmres_hand_trace_restart2_ends_its_cycle_unconverged():
+    """The same system with m = 2: jj=1 and jj=2 as in the m = 3 trace;
+    |gamma3| = 144/65 > eps, so v3 = w / 12 and the cycle ends with the back
+    substitution over restart_max columns (:783-806): alpha2 = (-12/13)/13 =
+    -12/169, alpha1 = (9/5 - (-17/25)(-12/169))/5 = 7401/21125.  The
+    residual after the cycle (the first history entry) is |b - A x1| =
+    sqrt(105339456/17850625).  (The later cycles of the reference's rotation
+    diverge on this system; only the first is traced.)"""
+    A, jac, b = gm3_system()
+    x = gm3_x([Fraction(7401, 21125), Fraction(-12, 169)])
+    assert x == [Fraction(43717, 42250), Fraction(23342, 21125), Fraction(24092, 21125)]
+    r = [Fraction(GM3_B[i]) - sum(Fraction(GM3_A[i][j]) * x[j] for j in range(3)) for i in range(3)]
+    r2 = sum(v * v for v in r)
+    # GM3_A and GM3_B hold binary roundings of thirds; the exact system's
+    # value is the one stated above
+    assert abs(float(r2) - 105339456 / 17850625) <= 1e-13
+    ref = O.gmres_jacobi(A, jac, b, eps=1e-10, m=2, iter_max=0)
+    assert ref["iters"] == 1 and ref["reason"] == 0 and ref["cycles"] == 3
+    want = (105339456 / 17850625) ** 0.5
+    assert abs(ref["hist"][0] - want) <= 1e-13 * want
 
 
 def test_coo_accumulate_adds_entry_by_entry():
