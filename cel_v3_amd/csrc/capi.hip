@@ -645,6 +645,20 @@ static int rebuild_halo(Mat& M)
     M.halo_nrows = (int)hrows.size();
     CEL_TRY(upload(M.halo_rows, hrows.data(), hrows.size()));
     CEL_TRY(upload(M.halo_rowq, rowq.data(), rowq.size()));
+    // (host arrays stay alive until the stream synchronisation below)
+    // per row its halo entry count (saturated at 255: the K1 only tells
+    // 0, 1, 2 and more) and a zero at [n], which the K1 reads for rows past
+    // the last one
+    std::vector<uint8_t> hflag((size_t)n + 1, 0);
+    for (int i = 0; i < n; ++i) hflag[i] = (uint8_t)std::min(rowcnt[i], 255);
+    CEL_TRY(upload(M.halo_flag, hflag.data(), hflag.size()));
+    // compact index of the first halo row of every 64-row group (one
+    // wavefront of a K1 tile), so a K1 wave finds its rows' records from
+    // one load and a ballot
+    std::vector<int> wq((size_t)(n + 63) / 64 + 1, 0);
+    for (int i = 0; i < n; ++i) wq[(size_t)i / 64 + 1] += rowcnt[i] ? 1 : 0;
+    for (size_t g = 1; g < wq.size(); ++g) wq[g] += wq[g - 1];
+    CEL_TRY(upload(M.halo_wq, wq.data(), wq.size()));
     CEL_TRY(upload(M.halo_ptr, hptr.data(), hptr.size()));
     CEL_TRY(upload(M.halo_cols, hcols.data(), hcols.size()));
     CEL_TRY(upload(M.halo_vals, hvals.data(), hvals.size()));

@@ -174,6 +174,8 @@ struct Mat {
     DevBuf<int> halo_xadj;         // CSR over local rows of the merged halo entries (n+1)
     DevBuf<int> halo_rows;         // rows having halo entries (compact list)
     DevBuf<int> halo_rowq;         // per local row: its compact index, or -1
+    DevBuf<uint8_t> halo_flag;     // per local row: min(halo entries, 255); [n] = 0
+    DevBuf<int> halo_wq;           // per 64-row group g: halo rows below row 64 g
     DevBuf<int> halo_ptr;          // per compact row: start into halo_cols/vals (+1 end)
     DevBuf<int> halo_cols;         // ghost slot per entry, in block order
     DevBuf<double> halo_vals;
@@ -235,7 +237,6 @@ int launch_dia_spmv_fused(const Mat& A, int fuse, long long chunk, int G, const 
 int dia_prepare_f32(Mat& A);
 // Sets bit `bit` (= dia_nd) of the DIA row masks on the rows with halo
 // entries and clears it elsewhere; fails when the masks have no spare bit.
-int dia_mark_halo_rows(Mat& A, int& bit);
 int launch_halo_spmv(const Mat& A, const double* ghost, double* y);
 int launch_face_pack(const Mat& A, const double* x, double* sendbuf);
 int launch_extract_diag(Mat& A);

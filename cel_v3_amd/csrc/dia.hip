@@ -168,47 +168,7 @@ k_dia_spmv_fused(int n, long long chunk, Offsets O, const double* __restrict__ d
 }
 
 
-// Halo-row flag (bit `bit` of the row masks): cleared on every row, then
-// set on the rows that have halo entries.
-__global__ void __launch_bounds__(kBlock)
-k_dia_flag_clear(int n, uint8_t* __restrict__ m8, uint32_t* __restrict__ m32, int bit)
-{
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        if (m8) m8[i] = (uint8_t)(m8[i] & ~(1u << bit));
-        else m32[i] = m32[i] & ~(1u << bit);
-    }
-}
-
-__global__ void __launch_bounds__(kBlock)
-k_dia_flag_set(int nrows, const int* __restrict__ rows, uint8_t* __restrict__ m8,
-               uint32_t* __restrict__ m32, int bit)
-{
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nrows) return;
-    const int i = rows[q];
-    if (m8) m8[i] = (uint8_t)(m8[i] | (1u << bit));
-    else m32[i] = m32[i] | (1u << bit);
-}
-
 }  // namespace
-
-int dia_mark_halo_rows(Mat& A, int& bit)
-{
-    const int cap = A.dia_mask8.p ? 8 : 32;
-    if (A.dia_nd == 0 || A.dia_nd >= cap)
-        return fail(CEL_ERR_STATE, "the DIA row masks have no spare bit for the halo rows");
-    bit = A.dia_nd;
-    if (A.n == 0) return CEL_OK;
-    hipStream_t st = rt().stream;
-    hipLaunchKernelGGL(k_dia_flag_clear, dim3(grid_of(A.n)), dim3(kBlock), 0, st, A.n, A.dia_mask8.p,
-                       A.dia_mask.p, bit);
-    if (A.halo_nrows > 0)
-        hipLaunchKernelGGL(k_dia_flag_set, dim3((A.halo_nrows + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                           st, A.halo_nrows, A.halo_rows.p, A.dia_mask8.p, A.dia_mask.p, bit);
-    CEL_HIP(hipGetLastError());
-    return CEL_OK;
-}
 
 int launch_dia_spmv(const Mat& A, const double* x, double* y, int accumulate, const int* skip)
 {
@@ -309,8 +269,14 @@ extern "C" int cel_gpu_mat_to_dia(cel_gpu_mat* A)
     CEL_HIP(hipStreamSynchronize(st));
     const size_t cnt = (size_t)nd * (size_t)M.n;
     CEL_TRY(M.dia_vals.alloc(cnt));
-    if (nd <= 8) CEL_TRY(M.dia_mask8.alloc(M.n));
-    else CEL_TRY(M.dia_mask.alloc(M.n));
+    // exactly one mask array: the kernels read both (the other reads 0)
+    if (nd <= 8) {
+        M.dia_mask.release();
+        CEL_TRY(M.dia_mask8.alloc(M.n));
+    } else {
+        M.dia_mask8.release();
+        CEL_TRY(M.dia_mask.alloc(M.n));
+    }
     CEL_HIP(hipMemsetAsync(M.dia_vals.p, 0, sizeof(double) * cnt, st));
     Offsets O{};
     O.nd = nd;

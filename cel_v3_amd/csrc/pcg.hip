@@ -173,13 +173,40 @@ struct MatView {
     const uint8_t* dmask8;   // presence bits, at most 8 diagonals
     int nd;
     int dzero;               // index of the diagonal with offset 0, or -1
-    int hbit;                // with halo: mask bit of the rows that have halo entries
     int doff[32];
     // halo (FP64 only)
     const int *hrow_tile, *hrows, *hrowq, *hptr, *hcols;
     const double* hvals;
     const double* ghost;
+    const double* hrec;     // per compact halo row: {p1, p2}
+    const uint8_t* hflag;   // per row: min(halo entries, 255); hflag[n] = 0
+    const int* hwq;         // per 64-row group: compact index of its first halo row
+    int hnq;                // rows with halo entries
 };
+
+// The halo records of this SpMV (after the ghost values arrived; here
+// after the face pack): one thread per row with halo entries, in compact
+// order.
+__global__ __launch_bounds__(kBlock) void k_halo_rec(int nq, const int* __restrict__ hptr,
+                                                     const int* __restrict__ hcols,
+                                                     const double* __restrict__ hvals,
+                                                     const double* __restrict__ ghost,
+                                                     double* __restrict__ rec,
+                                                     const Ctrl* __restrict__ cin)
+{
+    if (!cin->run) return;  // uniform: the iteration does not run
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int j0 = hptr[q];
+    const int c = hptr[q + 1] - j0;
+    double w[kHaloRec] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kHaloRec; ++k)
+        if (k < c) w[k] = hvals[j0 + k] * ghost[hcols[j0 + k]];
+    double* o = rec + 2 * (size_t)q;  // compact: the K1s find q from their wavefront
+    o[0] = w[0];
+    o[1] = w[1];
+}
 
 template <typename VT>
 __device__ __forceinline__ const VT* csr_vals(const MatView& A);
@@ -401,23 +428,34 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
     const long long ntiles = (A.n + kTileRows - 1) / kTileRows;
     double acc = 0.0;
     if constexpr (FMT == CEL_FMT_CSR) {
-        // the halo products of each tile are computed into LDS while the
-        // tile is staged and added after the local sum, in block order
-        __shared__ HaloStage hs;
+        // halo rows (HALO): a row's halo record (count and products, in
+        // block order) is added after its local sum.  Its flag loads one
+        // step ahead, behind the previous tile's entries; the record loads
+        // in the row's own step, behind the gathers (so the wait for the
+        // gathers does not include it) and ahead of the next tile's entries
+        // (whose wait, for the stage, covers it).  Unflagged rows load no
+        // record and read count 0.  Each loaded value is used once in the
+        // step after it, in place: a copy of a register still being loaded
+        // would wait for every load issued before it.
         CsrTileStream<VT> cs;
         const VT* vals = csr_vals<VT>(A);
-        bool hstaged = true;
-        if constexpr (HALO) {
-            halo_stage_init(hs);
-            __syncthreads();
-        }
         const long long tfirst = group_tile(0, ntiles);
+        const auto rrec = tile_rsrc(HALO ? A.hrec : nullptr, HALO ? A.hnq * 16 : 0);
+        // flag index of this thread's row of tile t (rows past n read hflag[n] = 0)
+        auto fidx = [&](long long t) {
+            const long long r = t * kTileRows + threadIdx.x;
+            return t >= 0 && r < A.n ? (int)r : A.n;
+        };
+        // the halo flag (entry count) of this thread's row of the tile being
+        // summed, and its wavefront's first compact halo index
+        int hf = 0, hq = 0;
         if (tfirst >= 0) {  // uniform across the block
             cs.start(A.n, (int)tfirst, (int)group_tile(1, ntiles), A.xadj, A.cols, vals, sm);
+            if constexpr (HALO) {
+                hf = A.hflag[fidx(tfirst)];
+                hq = A.hwq[fidx(tfirst) >> 6];
+            }
             cs.stage(sm);
-            if constexpr (HALO)
-                hstaged = halo_stage_mark(hs, (int)tfirst, A.hrow_tile, A.hrows, A.hptr, A.hcols,
-                                          A.hvals, A.ghost);
             __syncthreads();
         }
         long long k = 0;
@@ -426,17 +464,29 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
             const int aft = (int)group_tile(k + 2, ntiles);
             const int row = (int)t * kTileRows + threadIdx.x;
             VT pr = VT(0);
+            HaloRec hw{};
+            int hc = 0;
+            auto recs = [&]() {
+                if constexpr (HALO) {
+                    hc = hf;
+                    hw = halo_rec_load(rrec, hq, hc);
+                    hf = A.hflag[fidx(nxt)];
+                    hq = A.hwq[fidx(nxt) >> 6];
+                }
+            };
             VT s = cs.template rows<VT>(A.n, (int)t, nxt, aft, A.xadj, A.cols, vals, p, sm, row,
-                                        &pr);
+                                        &pr, VT(0), recs);
             s = A.coo ? s : VT(0) + s;
-            if constexpr (HALO)
-                s = halo_stage_add<VT>(hs, (int)t, hstaged, s, A.hcols, A.hvals, A.ghost);
-            __syncthreads();  // the tile's LDS stage and halo products are consumed
+            if constexpr (HALO) {
+                bool over = false;
+                const VT sh = halo_rec_add<VT>(s, hw, hc, over);
+                s = over ? halo_row<VT>(row, (int)t, s, A.hrow_tile, A.hrows, A.hptr, A.hcols, A.hvals,
+                                        A.ghost)
+                         : sh;
+            }
+            __syncthreads();  // the tile's LDS stage is consumed
             if (nxt >= 0) {
                 cs.stage(sm);
-                if constexpr (HALO)
-                    hstaged = halo_stage_mark(hs, nxt, A.hrow_tile, A.hrows, A.hptr, A.hcols,
-                                              A.hvals, A.ghost);
                 __syncthreads();
             }
             if (row < A.n) {
@@ -447,9 +497,46 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
         }
     } else if constexpr (FMT == CEL_FMT_DIA) {
         // all tiles of a group per step, one row of each per thread; the
-        // rows are added to the partial in tile order, as in the CSR K1
+        // rows are added to the partial in tile order, as in the CSR K1.
+        // Halo rows (HALO): as in the CSR K1, a row's record is added after
+        // its local sum.  The records of a step load at the end of the step
+        // before (after its products, into the registers those just freed)
+        // and the flags one step earlier still, so both are in by the time
+        // the step's products are and neither is copied while in flight.
         constexpr int R = kTileGroup;
         const long long ngroups = (ntiles + R - 1) / R;
+        const auto rrec = tile_rsrc(HALO ? A.hrec : nullptr, HALO ? A.hnq * 16 : 0);
+        // flag index of this thread's row of tile t (rows past n read hflag[n] = 0)
+        auto fidx = [&](long long t) {
+            const long long r = t * kTileRows + threadIdx.x;
+            return t < ntiles && r < A.n ? (int)r : A.n;
+        };
+        // flags and first compact halo indices of the next group, entry
+        // counts and records of this group
+        int hf[R], hq[R], hc[R];
+        HaloRec hr[R];
+        auto load_flags = [&](long long q) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                hf[k] = A.hflag[fidx(R * q + k)];
+                hq[k] = A.hwq[fidx(R * q + k) >> 6];
+            }
+        };
+        // a wave without halo rows in a tile skips that tile's record load
+        auto load_recs = [&]() {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                hc[k] = hf[k];
+                hr[k] = HaloRec{};
+                if (__builtin_amdgcn_ballot_w64(hc[k] != 0)) hr[k] = halo_rec_load(rrec, hq[k], hc[k]);
+            }
+        };
+        if constexpr (HALO) {
+            const long long q0 = blockIdx.x;
+            load_flags(q0);
+            load_recs();
+            load_flags(q0 + gridDim.x);
+        }
         for (long long q = blockIdx.x; q < ngroups; q += gridDim.x) {
             int r[R];
 #pragma unroll
@@ -468,16 +555,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
                         const VT pk = A.dzero >= 0 ? pr[k] : p[r[k]];
                         VT sk = VT(0) + s[k];
                         if constexpr (HALO) {
-                            // rows flagged in the masks add their halo
-                            // products after the local sum, in block order,
-                            // as the CSR K1 does
-                            const unsigned mk = A.dmask8 ? (unsigned)A.dmask8[r[k]] : A.dmask[r[k]];
-                            if ((mk >> A.hbit) & 1u) {
+                            bool over = false;
+                            sk = halo_rec_add<VT>(sk, hr[k], hc[k], over);
+                            if (over) {  // more than kHaloRec entries: the compact lists
+                                VT s0 = VT(0) + s[k];
                                 const int hq = A.hrowq[r[k]];
                                 for (int j = A.hptr[hq]; j < A.hptr[hq + 1]; ++j) {
                                     VT prod = VT(A.hvals[j]) * VT(A.ghost[A.hcols[j]]);
-                                    sk = sk + prod;
+                                    s0 = s0 + prod;
                                 }
+                                sk = s0;
                             }
                         }
                         v[r[k]] = sk;
@@ -485,6 +572,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(V
                         acc = acc + prod;
                     }
                 }
+            }
+            if constexpr (HALO) {  // the next group's records, the flags of the one after
+                load_recs();
+                load_flags(q + 2 * (long long)gridDim.x);
             }
         }
     } else {
@@ -756,6 +847,8 @@ struct Workspace {
     DevBuf<float> r32, p32, v32, e32;
     // JAD solves: b, x and the Jacobi vector in the permuted row order
     DevBuf<double> bp, xp, jacp;
+    // halo solves: per-row halo records {count, p1, p2, p3}
+    DevBuf<double> hrec;
     DevBuf<float> jacp32;
     DevBuf<Ctrl> ctrl;
     DevBuf<int> hrow_tile;
@@ -861,6 +954,8 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         return fail(CEL_ERR_STATE, "DIA requested but the matrix has no DIA form");
     if (halo && (mixed || fmt == CEL_FMT_JAD))
         return fail(CEL_ERR_ARG, "halo blocks are applied by the FP64 CSR and DIA forms only");
+    if (halo && (double)A.halo_nrows * 16.0 >= 2147483648.0 - 64.0)
+        return fail(CEL_ERR_ARG, "halo solve: the halo records need 31-bit offsets (< 2^27 halo rows)");
     if (!A.has_jac) return fail(CEL_ERR_STATE, "cel_gpu_mat_set_jacobi has not been called");
     if (A.ncols != n) return fail(CEL_ERR_ARG, "PCG needs a square local block");
     if (prm.iter_max < 0) return fail(CEL_ERR_ARG, "iter_max < 0");
@@ -925,9 +1020,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
                               (const void*)k_residual<float, CEL_FMT_JAD, false, true>}, ntiles);
         }
     }
-    if (A.halo.empty()) {
-        // (a block with halo blocks keeps every resident block: its K1s wait
-        // on the halo products' loads, which more blocks hide)
+    {
         const int cap = rt().num_cus * (mixed ? kK1BlocksPerCuF32 : kK1BlocksPerCuF64);
         if (G1 > cap) G1 = cap >= kXcds ? cap / kXcds * kXcds : cap;
     }
@@ -1030,10 +1123,9 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         CEL_TRY(W.ghost.alloc(A.ghost_len > 0 ? A.ghost_len : 1));
         CEL_TRY(W.sendbuf.alloc(A.send_total > 0 ? A.send_total : 1));
         hipLaunchKernelGGL(k_fill, dim3(64), dim3(kBlock), 0, s, (long long)W.ghost.n, W.ghost.p, 0.0);
+        CEL_TRY(W.hrec.alloc(2 * (size_t)std::max(A.halo_nrows, 1)));
         CEL_TRY(build_hrow_tile(A, W, ntiles));
     }
-    int hbit = 0;  // DIA with halo: mask bit that flags the rows with halo entries
-    if (halo && fmt == CEL_FMT_DIA) CEL_TRY(dia_mark_halo_rows(A, hbit));
 
     Prm P{};
     P.tol = prm.tol;
@@ -1077,7 +1169,6 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     V.dmask = A.dia_mask.p;
     V.dmask8 = A.dia_mask8.p;
     V.nd = A.dia_nd;
-    V.hbit = hbit;
     V.dzero = -1;
     for (int d = 0; d < A.dia_nd; ++d) {
         V.doff[d] = A.dia_off[d];
@@ -1091,6 +1182,10 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         V.hcols = A.halo_cols.p;
         V.hvals = A.halo_vals.p;
         V.ghost = Wp->ghost.p;
+        V.hrec = Wp->hrec.p;
+        V.hflag = A.halo_flag.p;
+        V.hwq = A.halo_wq.p;
+        V.hnq = A.halo_nrows;
     }
 
     // the allocation-free solve creates no events: it times with the host
@@ -1169,7 +1264,14 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
     const long long n_ll = n;
     auto enqueue_iter = [&](bool profile) -> int {
         hipEvent_t ea = nullptr, eb = nullptr;
-        if (halo) CEL_TRY(launch_face_pack(A, w.p, Wp->sendbuf.p));
+        if (halo) {
+            CEL_TRY(launch_face_pack(A, w.p, Wp->sendbuf.p));
+            // (a multi-GPU solve exchanges the faces here, then:)
+            if (A.halo_nrows > 0)
+                hipLaunchKernelGGL(k_halo_rec, dim3((A.halo_nrows + kBlock - 1) / kBlock), dim3(kBlock),
+                                   0, s, A.halo_nrows, A.halo_ptr.p, A.halo_cols.p,
+                                   A.halo_vals.p, Wp->ghost.p, Wp->hrec.p, C0);
+        }
         if (profile) {
             ea = prof_event();
             eb = prof_event();

@@ -71,6 +71,8 @@ __device__ __forceinline__ double fuse_row(double s, long long row, const double
     }
 }
 
+typedef unsigned int U32x4 __attribute__((ext_vector_type(4)));  // one 16-byte load
+
 template <typename VT>
 struct CsrStage {
     VT val[kStageCap];
@@ -201,6 +203,10 @@ __device__ __forceinline__ void dia_rows(int n, int nd, const int* doff, const V
         }
     }
 }
+
+struct NoLoads {
+    __device__ __forceinline__ void operator()() const {}
+};
 
 // Software-pipelined CSR tile stream.  bounds(t) loads the entry range of
 // tile t (two row pointers); fetch(t) issues the loads of tile t (its
@@ -333,11 +339,12 @@ struct CsrTileStream {
     // is not null) are issued first (see above).  The fetched registers are
     // consumed by the caller's stage() in the same iteration, never carried
     // across it.
-    template <typename AT>
+    template <typename AT, typename Extra = NoLoads>
     __device__ __forceinline__ AT rows(int n, int t, int next, int after,
                                        const int* __restrict__ xadj, const int* __restrict__ cols,
                                        const VT* __restrict__ vals, const VT* __restrict__ x,
-                                       const CsrStage<VT>& sm, int row, VT* xrow, AT init = AT(0))
+                                       const CsrStage<VT>& sm, int row, VT* xrow, AT init = AT(0),
+                                       Extra&& extra = Extra{})
     {
         const int b0 = e0, l = lo - e0, len = hi - lo;
         const bool staged = cnt <= kStageMax;  // uniform across the block
@@ -357,6 +364,10 @@ struct CsrTileStream {
         }
         VT xr = VT(0);
         if (xrow) xr = x[row < n ? row : 0];
+        // the caller's loads for this step: behind the gathers (waiting for
+        // those does not wait for these), ahead of the next tile's entries
+        // (whose wait, for the stage, covers these)
+        extra();
         // keep the issue order (gathers, then the next tile): the scheduler
         // would otherwise move the gathers next to their uses
         __builtin_amdgcn_sched_barrier(0);
@@ -518,6 +529,54 @@ struct JadStream {
         return sum;
     }
 };
+
+// Halo record of a row with halo entries: the products a*ghost of its
+// first two halo entries in block order, 16 bytes at its compact index q
+// (rows with halo entries in ascending order), written every SpMV by
+// k_halo_rec.  The row's entry count comes from its halo flag (hflag[row]
+// = min(count, 255)) and q from the wavefront: the first compact index of
+// its 64-row group (wq[row / 64]) plus the flagged lanes below it.  A K1
+// adds the products to the local sum one by one; a row with more than two
+// halo entries takes them from the compact lists instead.  The records are
+// read through a buffer resource over the compact rows (16 bytes each,
+// below 2^31 - 64 in all): a row without halo entries reads from outside
+// it, so the 16-byte load touches no memory and returns 0s.  Records of
+// neighbouring rows sit next to each other, so a tile's records stream
+// rather than touching one memory page per row.
+constexpr int kHaloRec = 2;
+
+struct HaloRec {
+    U32x4 a;
+};
+
+// q0: the compact index of the wavefront's first flagged row (wq of its
+// 64-row group); c: this lane's halo entry count
+__device__ __forceinline__ HaloRec halo_rec_load(__amdgpu_buffer_rsrc_t rr, int q0, int c)
+{
+    const unsigned long long b = __builtin_amdgcn_ballot_w64(c != 0);
+    const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+    const unsigned o = c != 0 ? (unsigned)(q0 + (int)below) * 16u : 0xFFFFFFC0u;
+    HaloRec h;
+    h.a = __builtin_amdgcn_raw_buffer_load_b128(rr, (int)o, 0, 0);
+    return h;
+}
+
+__device__ __forceinline__ double u2d(unsigned lo, unsigned hi)
+{
+    return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+
+// sum + the record's products in order for a row with c halo entries;
+// overflow: more than kHaloRec entries (the caller adds them from the
+// compact lists instead)
+template <typename AT>
+__device__ __forceinline__ AT halo_rec_add(AT sum, const HaloRec& h, int c, bool& overflow)
+{
+    overflow = c > kHaloRec;
+    if (c > 0) sum = sum + AT(u2d(h.a[0], h.a[1]));
+    if (c > 1) sum = sum + AT(u2d(h.a[2], h.a[3]));
+    return sum;
+}
 
 // Halo entries (rows of this rank with columns owned by a neighbour).
 // hrow_tile[t]..hrow_tile[t+1] index the compact, ascending list of rows of

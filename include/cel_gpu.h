@@ -339,9 +339,8 @@ void cel_gpu_pcg_params_default(cel_pcg_params* p);
  * (iter_max+1 doubles); for the mixed form it receives the inner estimate.
  * Requires cel_gpu_mat_set_jacobi.  With use_halo the halo blocks are
  * applied with a zero ghost region (one subdomain, no neighbours present)
- * and the send faces are packed every SpMV; FP64 only, in the CSR form or
- * in the DIA form (when its row masks have a spare bit: fewer than 8
- * diagonals, or fewer than 32 with the 32-bit masks). */
+ * and the send faces are packed every SpMV; FP64 only, in the CSR or the
+ * DIA form, with fewer than 2^27 rows that have halo entries. */
 int cel_gpu_pcg_solve(cel_gpu_mat* A, const double* b, double* x, const cel_pcg_params* params,
                       cel_pcg_stats* stats, double* hist);
 

@@ -116,11 +116,11 @@ def test_pcg_with_dia_is_bitwise_the_csr_solve(gpu, precision):
 
 def test_pcg_dia_with_halo_faces_is_bitwise_the_csr_solve(gpu):
     """The centre block of a 3x3x3 split (six halo faces, face packing every
-    iteration): the DIA K1 flags the rows with halo entries in a spare bit
-    of the row masks and adds their halo products after the local sum, in
-    block order, as the CSR K1 does.  The two halo solves, and the local
-    solve (the ghost region of one subdomain is zero), agree bit for bit;
-    the flag bit leaves the plain DIA product and solve unchanged."""
+    iteration; edge rows carry two halo entries, corner rows three, which
+    take the compact lists): the DIA K1 adds the halo products of a row
+    after its local sum, in block order, as the CSR K1 does.  The two halo
+    solves, and the local solve (the ghost region of one subdomain is
+    zero), agree bit for bit, and the plain DIA product is unchanged."""
     from test_gpu_distr import block_major_poisson
 
     m, d, rank = 12, 3, 13
