@@ -273,6 +273,29 @@ def sort_bench(cel, N, steps, warmup, skip_cpu):
     return out
 
 
+def bandwidth_bench(cel):
+    """HBM read rate of 2 GiB read as 1-8 equal arrays at once (the SpMV
+    kernels read 3-5 arrays together), 1-8 blocks of 256 threads per CU,
+    best of 3 means of 5 launches, and of a 1 GiB copy: the measured
+    ceilings beside the 8 TB/s nominal peak."""
+    nbytes = 2 * 1024 ** 3
+    out = {"read_bytes": nbytes, "read_GBs": {}}
+    for bpc in (1, 2, 4, 8):
+        for k in (1, 2, 3, 4, 5, 8):
+            ms = min(cel.stream_read_time(nbytes, streams=k, blocks_per_cu=bpc, reps=5)
+                     for _ in range(3))
+            out["read_GBs"][f"{k}_arrays_{bpc}_per_cu"] = nbytes / ms * 1e-6
+            log(f"read {k} arrays, {bpc} blocks/CU: {nbytes / ms * 1e-6:.0f} GB/s")
+    cb = 1024 ** 3
+    ms = min(cel.stream_copy_time(cb, reps=5) for _ in range(3))
+    out["copy_GBs"] = 2 * cb / ms * 1e-6
+    log(f"copy: {out['copy_GBs']:.0f} GB/s (read + write)")
+    best = max(out["read_GBs"].values())
+    out["read_GBs_best"] = best
+    out["read_frac_of_8TBs"] = best / 8000.0
+    return out
+
+
 def sort_cpu_baseline():
     """The oracle's restatement of the reference's COO sort
     (orc_coo_sort_cel: cel_sp_mat_format_sparse_31_sort, one thread, as the
@@ -382,7 +405,7 @@ def main():
     ap.add_argument("--skip-config3", action="store_true", help="do not run the 384^3 workload")
     ap.add_argument("--only", default="all",
                     choices=["all", "fp64", "mixed", "spmv", "profile", "batch", "sort", "gmres",
-                             "config3"],
+                             "config3", "bandwidth"],
                     help="run one part only; 'profile' = one streaming copy of known size "
                          "(counter calibration) and FP64, FP64-with-halo and mixed solves "
                          "capped at --iters iterations in every form, for hardware-counter "
@@ -391,7 +414,8 @@ def main():
                          "COO ordering of the shuffled --size^3 matrix in HBM; 'gmres' = "
                          "Jacobi-GMRES(3) with the element solver's settings on the 128^3 block "
                          "(--iters > 0 caps its outer iterations); 'config3' = the 384^3 "
-                         "precision workload")
+                         "precision workload; 'bandwidth' = HBM read rate of 1-8 arrays read "
+                         "together and of a copy (the measured ceiling beside the 8 TB/s peak)")
     ap.add_argument("--iters", type=int, default=None,
                     help="iteration cap of --only profile (default 20) and --only gmres (default none)")
     args = ap.parse_args()
@@ -414,6 +438,10 @@ def main():
     if args.only == "gmres":
         cel.init(0)
         print(json.dumps(gmres_bench(cel, 128, args.steps, args.iters or 0)))
+        return
+    if args.only == "bandwidth":
+        cel.init(0)
+        print(json.dumps(bandwidth_bench(cel)))
         return
     if args.only == "sort":
         cel.init(0)
@@ -586,6 +614,8 @@ def main():
         for form in ("csr", "dia"):
             solve(cel.PREC_FP64, form, halo=1, mat=H)  # warm-up
             halo_timed[form] = solve(cel.PREC_FP64, form, halo=1, mat=H)
+            # K1 of the same handle without and with its halo blocks, back to back
+            prof["fp64_hlocal_" + form] = solve(cel.PREC_FP64, form, profile=1, mat=H)
             prof["fp64_halo_" + form] = solve(cel.PREC_FP64, form, profile=1, halo=1, mat=H)
         log("fp64 with halo faces: " + ", ".join(f"{k} {v['solve_ms']:.1f} ms" for k, v in halo_timed.items()))
         halo_x = {k: v["x"].to_host() for k, v in halo_timed.items()}
@@ -674,7 +704,14 @@ def main():
                             "note": "one-subdomain model: 6 halo blocks and face packing every "
                                     "SpMV, received values zero",
                             "solutions_bitwise_equal_to_local_csr_solve": {
-                                k: bool(np.array_equal(v, halo_local)) for k, v in halo_x.items()}}
+                                k: bool(np.array_equal(v, halo_local)) for k, v in halo_x.items()},
+                            # mean K1 launch (us) of the same handle without and with its
+                            # halo blocks, back to back
+                            "k1_us_local_vs_halo": {
+                                f: [k1(prof["fp64_hlocal_" + f])[0] * 1e3,
+                                    k1(prof["fp64_halo_" + f])[0] * 1e3] for f in ("csr", "dia")}}
+        for f, (a, b) in res["fp64_halo"]["k1_us_local_vs_halo"].items():
+            log(f"halo K1 {f}: {b:.1f} us against {a:.1f} us without halo ({b / a:.3f}x)")
     res["config"] = cfg
     res.update(out)
     if not args.skip_config3 and args.only == "all":

@@ -45,7 +45,8 @@ EXPORTS = [
     "cel_gpu_mat_get_block", "cel_gpu_mat_set_send_list", "cel_gpu_face_pack",
     "cel_gpu_spmv_distr", "cel_gpu_mat_ghost_len", "cel_gpu_pcg_params_default",
     "cel_gpu_pcg_solve", "cel_gpu_cg_adjoint", "cel_gpu_diffusion7_kappa_grad",
-    "cel_gpu_mat_add_halo_block", "cel_gpu_stream_copy_time", "cel_gpu_batch_create",
+    "cel_gpu_mat_add_halo_block", "cel_gpu_stream_copy_time", "cel_gpu_stream_read_time",
+    "cel_gpu_batch_create",
     "cel_gpu_batch_destroy", "cel_gpu_batch_tune", "cel_gpu_batch_info", "cel_gpu_batch_solve",
     "cel_gpu_mm_info", "cel_gpu_mm_read", "cel_gpu_coo_sort", "cel_gpu_coo_row_offsets",
     "cel_gpu_mat_create_coo_sorted", "cel_gpu_gmres_params_default", "cel_gpu_gmres_solve",
@@ -717,6 +718,13 @@ class Elm:
 def stream_copy_time(nbytes, reps=20):
     ms = C.c_double()
     _check(lib().cel_gpu_stream_copy_time(C.c_size_t(nbytes), reps, C.byref(ms)))
+    return ms.value
+
+
+def stream_read_time(nbytes, streams=1, blocks_per_cu=4, reps=10):
+    ms = C.c_double()
+    _check(lib().cel_gpu_stream_read_time(C.c_size_t(nbytes), streams, blocks_per_cu, reps,
+                                          C.byref(ms)))
     return ms.value
 
 

@@ -832,6 +832,67 @@ extern "C" int cel_gpu_stream_copy_time(size_t bytes, int reps, double* mean_ms)
     return CEL_OK;
 }
 
+// K arrays of n16 16-byte units (stride16 apart) read together; the xor of
+// the words is kept live by a store that never happens (the sum of the
+// read words cannot equal the impossible value), so nothing is written.
+typedef unsigned ReadVec __attribute__((ext_vector_type(4)));
+template <int K>
+__global__ __launch_bounds__(256) void k_stream_read(const ReadVec* __restrict__ a, size_t n16,
+                                                     size_t stride16, unsigned* __restrict__ out)
+{
+    unsigned acc = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (size_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const ReadVec v = __builtin_nontemporal_load(a + k * stride16 + i);
+            acc ^= v.x ^ v.y ^ v.z ^ v.w;
+        }
+    }
+    if (acc == 0x9E3779B9u && blockIdx.x == 0x7FFFFFFF) out[0] = acc;
+}
+
+extern "C" int cel_gpu_stream_read_time(size_t bytes, int streams, int blocks_per_cu, int reps,
+                                        double* mean_ms)
+{
+    if (reps < 1 || streams < 1 || streams > 8 || blocks_per_cu < 1 || blocks_per_cu > 32)
+        return fail(CEL_ERR_ARG, "bad arguments to cel_gpu_stream_read_time");
+    CEL_TRY(ensure_init());
+    const size_t n16 = bytes / 16 / (size_t)streams;
+    if (n16 == 0) return fail(CEL_ERR_ARG, "cel_gpu_stream_read_time needs 16 bytes per stream");
+    DevBuf<ReadVec> a;
+    DevBuf<unsigned> out;
+    CEL_TRY(a.alloc(n16 * (size_t)streams));
+    CEL_TRY(out.alloc(1));
+    hipStream_t s = rt().stream;
+    CEL_HIP(hipMemsetAsync(a.p, 0, n16 * (size_t)streams * 16, s));
+    const int G = rt().num_cus * blocks_per_cu;
+    auto launch = [&]() {
+        switch (streams) {
+#define CEL_READ_CASE(K)                                                                           \
+    case K:                                                                                        \
+        hipLaunchKernelGGL(k_stream_read<K>, dim3(G), dim3(256), 0, s, a.p, n16, n16, out.p);     \
+        break;
+            CEL_READ_CASE(1) CEL_READ_CASE(2) CEL_READ_CASE(3) CEL_READ_CASE(4)
+            CEL_READ_CASE(5) CEL_READ_CASE(6) CEL_READ_CASE(7) CEL_READ_CASE(8)
+#undef CEL_READ_CASE
+        }
+    };
+    launch();
+    StreamScope R(s);
+    hipEvent_t e0, e1;
+    CEL_TRY(R.event(e0));
+    CEL_TRY(R.event(e1));
+    CEL_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i) launch();
+    CEL_HIP(hipEventRecord(e1, s));
+    CEL_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    CEL_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (mean_ms) *mean_ms = ms / reps;
+    return CEL_OK;
+}
+
 extern "C" int cel_gpu_mat_get_block(const cel_gpu_mat* A, int b, int* proc, int* nnz, int* rows,
                                      int* cols, double* vals, int* xadj, int* recv_len,
                                      int* recv_perm)

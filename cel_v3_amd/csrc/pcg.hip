@@ -644,6 +644,14 @@ __device__ __forceinline__ void set_lane(typename Vec<VT>::T& a, int k, VT v)
     }
 }
 
+// Whole 16-byte units per thread and loop step in the vector kernels (all
+// their loads in flight together).  Measured on the 256^3 solves (two
+// interleaved repetitions, one box): FP32 4 (mixed CSR 304-305 ms against
+// 309-310 with 1, 308-310 with 2), FP64 2 (CSR 497-501 against 503-507
+// with 1, 501-506 with 4).
+template <typename VT>
+constexpr int kVecUnrollOf = sizeof(VT) == 4 ? 4 : 2;
+
 // K2: sigma = alpha / sum(v.p) ; r = r + (-sigma) v ; z = jac*r ;
 // per-block partials of r.z and r.r.
 template <typename VT>
@@ -681,7 +689,35 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_r(long long n, VT* __rest
     const long long lo = u0 * unit, hi = min(u1 * unit, n);
     const VT ms = VT(-sigma);
     double arz = 0.0, arr = 0.0;
-    for (long long i = lo + (long long)threadIdx.x * W; i < hi; i += unit) {
+    constexpr int kVecUnroll = kVecUnrollOf<VT>;
+    long long i = lo + (long long)threadIdx.x * W;
+    // kVecUnroll units per step while they are whole vectors (all their
+    // loads in flight together), added to the partials in unit order
+    for (; i + (kVecUnroll - 1) * unit + W <= hi; i += kVecUnroll * unit) {
+        V vv[kVecUnroll], rv[kVecUnroll], jv[kVecUnroll];
+#pragma unroll
+        for (int u = 0; u < kVecUnroll; ++u) {
+            vv[u] = *reinterpret_cast<const V*>(v + i + u * unit);
+            rv[u] = *reinterpret_cast<const V*>(r + i + u * unit);
+            jv[u] = *reinterpret_cast<const V*>(jac + i + u * unit);
+        }
+#pragma unroll
+        for (int u = 0; u < kVecUnroll; ++u) {
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                VT t = ms * lane<VT>(vv[u], k);
+                VT ri = lane<VT>(rv[u], k) + t;
+                set_lane<VT>(rv[u], k, ri);
+                VT zi = ri * lane<VT>(jv[u], k);
+                double rz = double(ri) * double(zi);
+                arz = arz + rz;
+                double rr = double(ri) * double(ri);
+                arr = arr + rr;
+            }
+            *reinterpret_cast<V*>(r + i + u * unit) = rv[u];
+        }
+    }
+    for (; i < hi; i += unit) {
         if (i + W <= n) {
             const V vv = *reinterpret_cast<const V*>(v + i);
             V rv = *reinterpret_cast<const V*>(r + i);
@@ -776,7 +812,45 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xp(long long n, XT* __res
     long long u0, u1;
     unit_range((n + unit - 1) / unit, gridDim.x, lb, u0, u1);
     const long long lo = u0 * unit, hi = min(u1 * unit, n);
-    for (long long i = lo + (long long)threadIdx.x * W; i < hi; i += unit) {
+    constexpr int kVecUnroll = kVecUnrollOf<VT>;
+    long long i = lo + (long long)threadIdx.x * W;
+    // kVecUnroll units per step while they are whole vectors
+    for (; i + (kVecUnroll - 1) * unit + W <= hi; i += kVecUnroll * unit) {
+        V pv[kVecUnroll], rv[kVecUnroll], jv[kVecUnroll];
+        XT xs[kVecUnroll][W];
+#pragma unroll
+        for (int u = 0; u < kVecUnroll; ++u) {
+            pv[u] = *reinterpret_cast<const V*>(p + i + u * unit);
+            rv[u] = *reinterpret_cast<const V*>(r + i + u * unit);
+            jv[u] = *reinterpret_cast<const V*>(jac + i + u * unit);
+#pragma unroll
+            for (int k = 0; k < W; ++k) xs[u][k] = x[i + u * unit + k];
+        }
+#pragma unroll
+        for (int u = 0; u < kVecUnroll; ++u) {
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                const VT pi = lane<VT>(pv[u], k);
+                XT t = XT(sigma * pi);
+                xs[u][k] = xs[u][k] + t;
+                VT zi = lane<VT>(rv[u], k) * lane<VT>(jv[u], k);
+                VT bp = beta * pi;
+                set_lane<VT>(pv[u], k, bp + zi);
+            }
+            if (flush) {
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    double t = (double)xs[u][k];
+                    x64[i + u * unit + k] = x64[i + u * unit + k] + t;
+                    xs[u][k] = XT(0);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < W; ++k) x[i + u * unit + k] = xs[u][k];
+            *reinterpret_cast<V*>(p + i + u * unit) = pv[u];
+        }
+    }
+    for (; i < hi; i += unit) {
         if (i + W <= n) {
             V pv = *reinterpret_cast<const V*>(p + i);
             const V rv = *reinterpret_cast<const V*>(r + i);

@@ -330,6 +330,12 @@ int cel_gpu_mat_ghost_len(const cel_gpu_mat* A, int* ghost_len, int* send_total)
  * measured bandwidth ceiling next to the 8 TB/s nominal peak.  No reference
  * counterpart. */
 int cel_gpu_stream_copy_time(size_t bytes, int reps, double* mean_ms);
+/* Mean time of reading `bytes` from HBM as `streams` (1-8) equal arrays read
+ * together (16 bytes per lane and array per step, blocks_per_cu blocks of
+ * 256 threads on every CU), for the read-bandwidth ceiling of the SpMV
+ * kernels, which read several arrays at once.  No reference counterpart. */
+int cel_gpu_stream_read_time(size_t bytes, int streams, int blocks_per_cu, int reps,
+                             double* mean_ms);
 
 /* ---- Jacobi-PCG -------------------------------------------------------- */
 void cel_gpu_pcg_params_default(cel_pcg_params* p);
