@@ -46,6 +46,7 @@ EXPORTS = [
     "cel_gpu_spmv_distr", "cel_gpu_mat_ghost_len", "cel_gpu_pcg_params_default",
     "cel_gpu_pcg_solve", "cel_gpu_cg_adjoint", "cel_gpu_diffusion7_kappa_grad",
     "cel_gpu_mat_add_halo_block", "cel_gpu_stream_copy_time", "cel_gpu_stream_read_time",
+    "cel_gpu_spmv_dot",
     "cel_gpu_batch_create",
     "cel_gpu_batch_destroy", "cel_gpu_batch_tune", "cel_gpu_batch_info", "cel_gpu_batch_solve",
     "cel_gpu_mm_info", "cel_gpu_mm_read", "cel_gpu_coo_sort", "cel_gpu_coo_row_offsets",
@@ -405,6 +406,15 @@ class Matrix:
         g = _f64(ghost) if len(ghost) else np.zeros(1)
         _check(lib().cel_gpu_spmv_distr(self.h, fmt, _ptr(_f64(x)), _ptr(g), _ptr(y), MEM_HOST))
         return y
+
+    def spmv_dot(self, x, ghost=None, fmt=FMT_CSR):
+        """The PCG's fused product (host arrays): returns (y, x.y); with ghost,
+        the halo blocks are applied to those received values."""
+        y = np.zeros(self.n)
+        d = C.c_double()
+        g = None if ghost is None else _ptr(_f64(ghost) if len(ghost) else np.zeros(1))
+        _check(lib().cel_gpu_spmv_dot(self.h, fmt, _ptr(_f64(x)), g, _ptr(y), C.byref(d), MEM_HOST))
+        return y, d.value
 
     # solvers ------------------------------------------------------------
     def pcg(self, b, x0=None, tol=1e-8, tol_mode=TOL_ABS, iter_max=1000, x0_mode=X0_ONE,

@@ -11,6 +11,7 @@
 
 namespace cel {
 
+int pcg_spmv_dot(Mat& A, int fmt, const double* x, const double* ghost, double* y, double* dot);
 int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel_pcg_stats& stats,
               double* hist, bool noalloc = false);
 int pcg_noalloc_setup(int device);
@@ -529,6 +530,41 @@ extern "C" int cel_gpu_spmv_distr(cel_gpu_mat* A, int format, const double* x, c
     if (!A || (!x && A->m.ncols) || (!y && A->m.n) || (!ghost && A->m.ghost_len))
         return fail(CEL_ERR_ARG, "null argument");
     return spmv_impl<double>(A->m, format, x, y, 0, memspace, ghost, true);
+}
+
+extern "C" int cel_gpu_spmv_dot(cel_gpu_mat* A, int format, const double* x, const double* ghost,
+                                double* y, double* dot, int memspace)
+{
+    if (!A || (!x && A->m.n) || (!y && A->m.n) || !dot) return fail(CEL_ERR_ARG, "null argument");
+    if (memspace != CEL_MEM_HOST && memspace != CEL_MEM_DEVICE) return fail(CEL_ERR_ARG, "bad memspace");
+    CEL_TRY(ensure_init());
+    Mat& M = A->m;
+    if (ghost && M.halo.empty()) return fail(CEL_ERR_STATE, "ghost values given but no halo blocks");
+    const double* dx = x;
+    double* dy = y;
+    const double* dg = ghost;
+    DevBuf<double> bx, by, bg;
+    hipStream_t s = rt().stream;
+    if (memspace == CEL_MEM_HOST) {
+        CEL_TRY(upload(bx, x, (size_t)M.n));
+        CEL_TRY(by.alloc(M.n > 0 ? M.n : 1));
+        dx = bx.p;
+        dy = by.p;
+        if (ghost) {
+            CEL_TRY(upload(bg, ghost, (size_t)std::max(M.ghost_len, 1)));
+            dg = bg.p;
+        }
+    } else if (ghost && M.ghost_len == 0) {
+        // a device ghost region of length 0: the kernels still read through it
+        CEL_TRY(bg.alloc(1));
+        dg = bg.p;
+    }
+    CEL_TRY(pcg_spmv_dot(M, format, dx, dg, dy, dot));
+    if (memspace == CEL_MEM_HOST && M.n) {
+        CEL_HIP(hipMemcpyAsync(y, dy, sizeof(double) * M.n, hipMemcpyDeviceToHost, s));
+        CEL_HIP(hipStreamSynchronize(s));
+    }
+    return CEL_OK;
 }
 
 extern "C" int cel_gpu_spmv_time(cel_gpu_mat* A, int format, int precision, const void* x, void* y,

@@ -976,6 +976,14 @@ static int common_grid(std::initializer_list<const void*> kernels, long long wor
     return g;
 }
 
+// The K1 grid: at most kK1BlocksPerCu* blocks per CU, a multiple of the XCDs.
+static int k1_cap(int G1, bool mixed)
+{
+    const int cap = rt().num_cus * (mixed ? kK1BlocksPerCuF32 : kK1BlocksPerCuF64);
+    if (G1 > cap) G1 = cap >= kXcds ? cap / kXcds * kXcds : cap;
+    return G1;
+}
+
 // The allocation-free entry point's device memory: a file-scope array
 // (one copy per GPU, resolved to a device address once per device), laid
 // out as r, p, v (kNoAllocMaxRows doubles each), the three partial-sum
@@ -1007,6 +1015,102 @@ struct WorkPtrs {
     float *r32 = nullptr, *p32 = nullptr, *v32 = nullptr, *e32 = nullptr;
     Ctrl* ctrl = nullptr;
 };
+
+// One FP64 K1 of the solve on caller vectors (device pointers, n rows):
+// y = A x in the CSR or DIA form, plus, with ghost, the halo products of
+// the caller's ghost values (the halo records of the solve, built from
+// them first), and dot = x.y summed over the K1's block partials in
+// logical-block order, as the solve's K2 sums them.
+int pcg_spmv_dot(Mat& A, int fmt, const double* x, const double* ghost, double* y, double* dot)
+{
+    CEL_TRY(ensure_init());
+    const int n = A.n;
+    if (fmt != CEL_FMT_CSR && fmt != CEL_FMT_DIA)
+        return fail(CEL_ERR_ARG, "the fused product runs in the CSR or the DIA form");
+    if (fmt == CEL_FMT_DIA && A.dia_nd == 0)
+        return fail(CEL_ERR_STATE, "DIA requested but the matrix has no DIA form");
+    if (A.ncols != n) return fail(CEL_ERR_ARG, "the fused product needs a square local block");
+    const bool halo = ghost != nullptr;
+    if (halo && A.halo.empty()) return fail(CEL_ERR_STATE, "ghost values given but no halo blocks");
+    if (halo && (double)A.halo_nrows * 16.0 >= 2147483648.0 - 64.0)
+        return fail(CEL_ERR_ARG, "the halo records need 31-bit offsets (< 2^27 halo rows)");
+    if (fmt == CEL_FMT_DIA && (double)n * 8.0 >= 2147483648.0)
+        return fail(CEL_ERR_ARG, "the DIA kernels address x with 32-bit byte offsets");
+    if (dot) *dot = 0.0;
+    if (n == 0) return CEL_OK;
+    const int ntiles = (n + kTileRows - 1) / kTileRows;
+    const int G1 = k1_cap(common_grid({(const void*)k_pcg_spmv<double, CEL_FMT_CSR, false>,
+                                       (const void*)k_pcg_spmv<double, CEL_FMT_CSR, true>,
+                                       (const void*)k_pcg_spmv<double, CEL_FMT_DIA, false>,
+                                       (const void*)k_pcg_spmv<double, CEL_FMT_DIA, true>},
+                                      ntiles),
+                          false);
+    Workspace& W = ws_of(A);
+    CEL_TRY(W.part1.alloc((size_t)G1));
+    CEL_TRY(W.ctrl.alloc(5));
+    if (halo) {
+        CEL_TRY(W.hrec.alloc(2 * (size_t)std::max(A.halo_nrows, 1)));
+        // the CSR K1 finds the entries of rows with more than kHaloRec halo
+        // entries through the per-tile compact row ranges
+        CEL_TRY(build_hrow_tile(A, W, ntiles));
+    }
+    hipStream_t s = rt().stream;
+    Ctrl c0{};
+    c0.run = 1;
+    CEL_HIP(hipMemcpyAsync(W.ctrl.p, &c0, sizeof(Ctrl), hipMemcpyHostToDevice, s));
+    MatView V{};
+    V.n = n;
+    V.xadj = A.xadj.p;
+    V.cols = A.cols.p;
+    V.vals = A.vals.p;
+    V.dvals = A.dia_vals.p;
+    V.dmask = A.dia_mask.p;
+    V.dmask8 = A.dia_mask8.p;
+    V.nd = A.dia_nd;
+    V.dzero = -1;
+    for (int d = 0; d < A.dia_nd; ++d) {
+        V.doff[d] = A.dia_off[d];
+        if (A.dia_off[d] == 0) V.dzero = d;
+    }
+    if (halo) {
+        V.hrow_tile = W.hrow_tile.p;
+        V.hrows = A.halo_rows.p;
+        V.hptr = A.halo_ptr.p;
+        V.hcols = A.halo_cols.p;
+        V.hvals = A.halo_vals.p;
+        V.ghost = ghost;
+        V.hrec = W.hrec.p;
+        V.hrowq = A.halo_rowq.p;
+        V.hflag = A.halo_flag.p;
+        V.hwq = A.halo_wq.p;
+        V.hnq = A.halo_nrows;
+        if (A.halo_nrows > 0)
+            hipLaunchKernelGGL(k_halo_rec, dim3((A.halo_nrows + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                               A.halo_nrows, A.halo_ptr.p, A.halo_cols.p, A.halo_vals.p, ghost, W.hrec.p,
+                               W.ctrl.p);
+    }
+    const Prm P{};
+    if (fmt == CEL_FMT_CSR && halo)
+        hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_CSR, true>), dim3(G1), dim3(kBlock), 0, s, V, x, y,
+                           W.part1.p, nullptr, 0, W.ctrl.p, W.ctrl.p + 1, P);
+    else if (fmt == CEL_FMT_CSR)
+        hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_CSR, false>), dim3(G1), dim3(kBlock), 0, s, V, x, y,
+                           W.part1.p, nullptr, 0, W.ctrl.p, W.ctrl.p + 1, P);
+    else if (halo)
+        hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_DIA, true>), dim3(G1), dim3(kBlock), 0, s, V, x, y,
+                           W.part1.p, nullptr, 0, W.ctrl.p, W.ctrl.p + 1, P);
+    else
+        hipLaunchKernelGGL((k_pcg_spmv<double, CEL_FMT_DIA, false>), dim3(G1), dim3(kBlock), 0, s, V, x, y,
+                           W.part1.p, nullptr, 0, W.ctrl.p, W.ctrl.p + 1, P);
+    CEL_HIP(hipGetLastError());
+    std::vector<double> part((size_t)G1);
+    CEL_HIP(hipMemcpyAsync(part.data(), W.part1.p, sizeof(double) * G1, hipMemcpyDeviceToHost, s));
+    CEL_HIP(hipStreamSynchronize(s));
+    double acc = 0.0;
+    for (int b = 0; b < G1; ++b) acc = acc + part[(size_t)b];
+    if (dot) *dot = acc;
+    return CEL_OK;
+}
 
 int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel_pcg_stats& stats,
               double* hist, bool noalloc)
@@ -1094,10 +1198,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
                               (const void*)k_residual<float, CEL_FMT_JAD, false, true>}, ntiles);
         }
     }
-    {
-        const int cap = rt().num_cus * (mixed ? kK1BlocksPerCuF32 : kK1BlocksPerCuF64);
-        if (G1 > cap) G1 = cap >= kXcds ? cap / kXcds * kXcds : cap;
-    }
+    G1 = k1_cap(G1, mixed);
     {
         const long long units = ((long long)n + 2 * kBlock - 1) / (2 * kBlock);
         Gv = (int)std::min<long long>((long long)rt().num_cus * kVecBlocksPerCu,

@@ -582,80 +582,6 @@ __device__ __forceinline__ AT halo_rec_add(AT sum, const HaloRec& h, int c, bool
 // hrow_tile[t]..hrow_tile[t+1] index the compact, ascending list of rows of
 // tile t that have halo entries; entries of compact row q are
 // hptr[q]..hptr[q+1] in block order, and their columns are ghost slots.
-//
-// HaloStage holds, per block, the halo part of the tile being summed:
-// while tile t is staged (a phase that waits for the tile's loads anyway)
-// the block computes every halo product hvals[j] * ghost[hcols[j]] of the
-// tile into LDS and marks, per tile row, its tag (= t), first product and
-// count.  After the staging barrier each thread adds its row's products
-// from LDS in block order -- no global loads in the row epilogue.  Tags
-// make clearing unnecessary (a stale entry carries an older tile number).
-// A tile with more than kHaloStage halo entries reads them from global
-// memory instead (same order, same products).
-constexpr int kHaloStage = 1024;
-
-struct HaloStage {
-    double prod[kHaloStage];
-    int tag[kTileRows];
-    int first[kTileRows];  // LDS slot, or global entry index for an unstaged tile
-    int count[kTileRows];
-};
-
-// All threads; call once before the first mark, followed by a barrier.
-__device__ __forceinline__ void halo_stage_init(HaloStage& hs) { hs.tag[threadIdx.x] = -1; }
-
-// Stages tile t's halo products and row marks (before the barrier that
-// precedes the tile's rows).  Returns whether the products are in LDS
-// (uniform across the block).
-__device__ __forceinline__ bool halo_stage_mark(HaloStage& hs, int t,
-                                                const int* __restrict__ hrow_tile,
-                                                const int* __restrict__ hrows,
-                                                const int* __restrict__ hptr,
-                                                const int* __restrict__ hcols,
-                                                const double* __restrict__ hvals,
-                                                const double* __restrict__ ghost)
-{
-    const int r0 = t * kTileRows;
-    const int q0 = hrow_tile[t], q1 = hrow_tile[t + 1];
-    if (q0 == q1) return true;
-    const int e0 = hptr[q0], e1 = hptr[q1];
-    const bool staged = e1 - e0 <= kHaloStage;
-    if (staged) {
-        for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlock) {
-            double prod = hvals[e] * ghost[hcols[e]];
-            hs.prod[e - e0] = prod;
-        }
-    }
-    for (int q = q0 + (int)threadIdx.x; q < q1; q += kBlock) {
-        const int i = hrows[q] - r0;
-        const int j = hptr[q];
-        hs.tag[i] = t;
-        hs.first[i] = staged ? j - e0 : j;
-        hs.count[i] = hptr[q + 1] - j;
-    }
-    return staged;
-}
-
-// sum + the calling thread's halo products of tile t, in block order.
-template <typename AT>
-__device__ __forceinline__ AT halo_stage_add(const HaloStage& hs, int t, bool staged, AT sum,
-                                             const int* __restrict__ hcols,
-                                             const double* __restrict__ hvals,
-                                             const double* __restrict__ ghost)
-{
-    if (hs.tag[threadIdx.x] != t) return sum;
-    const int f = hs.first[threadIdx.x], c = hs.count[threadIdx.x];
-    for (int k = 0; k < c; ++k) {
-        AT prod;
-        if (staged) {
-            prod = AT(hs.prod[f + k]);
-        } else {
-            prod = AT(hvals[f + k]) * AT(ghost[hcols[f + k]]);
-        }
-        sum = sum + prod;
-    }
-    return sum;
-}
 
 // Halo entries of row `row` of tile t found by binary search in the tile's
 // compact rows (no LDS map; used where a kernel runs once per solve).

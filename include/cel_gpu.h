@@ -324,6 +324,17 @@ int cel_gpu_face_pack(cel_gpu_mat* A, const double* x, double* sendbuf, int mems
  * reference: local row sum first, then halo entries block by block. */
 int cel_gpu_spmv_distr(cel_gpu_mat* A, int format, const double* x, const double* ghost,
                        double* y, int memspace);
+/* The Jacobi-PCG's fused product on caller vectors: y = A_loc x in the CSR
+ * or DIA form plus, when ghost is not NULL, the halo blocks' products of the
+ * ghost values (as cel_gpu_spmv_distr: local row sum first, then the halo
+ * entries in block order), and *dot = x . y over the local rows.  It runs
+ * the kernel the FP64 solve runs once per iteration (the reference's
+ * v = A p and the v.p dot of cel_fop_vec_distr_alpha_allreduce in
+ * cel_cgalg_jacobi, src/fcgalg/cel_cgalg_module.f90:250-261), so y is
+ * bitwise the product of cel_gpu_spmv_distr and dot is summed in the
+ * solve's fixed order.  FP64, square local block. */
+int cel_gpu_spmv_dot(cel_gpu_mat* A, int format, const double* x, const double* ghost, double* y,
+                     double* dot, int memspace);
 int cel_gpu_mat_ghost_len(const cel_gpu_mat* A, int* ghost_len, int* send_total);
 
 /* Mean time of a streaming device copy of `bytes` (read + write), for the

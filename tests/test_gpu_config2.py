@@ -177,3 +177,24 @@ def test_mixed_reaches_tolerance_near_the_oracle(gpu, c2, fmt):
     eo = np.abs(xo - c2["xt"]).max()
     assert np.abs(x - c2["xt"]).max() <= 2.0 * eo
     assert np.abs(x - xo).max() <= 2.0 * eo
+
+
+def test_fused_product_of_the_six_face_block_with_ghost_values(gpu):
+    """The 256^3 block with its six halo faces (face, edge and corner rows:
+    1, 2 and 3 halo entries) and random received values: the solve's fused
+    product (cel_gpu_spmv_dot, CSR and DIA) is bitwise the distributed
+    product (cel_gpu_spmv_distr, pinned to the oracle in test_gpu_distr),
+    and its x.y agrees with numpy's to 1e-13 of sum |x_i y_i|."""
+    H = gpu.Matrix.poisson7(N, N, N)
+    _face_blocks(H)
+    H.to_dia()
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=H.n)
+    ghost = rng.normal(size=6 * N * N)
+    y = H.spmv_distr(x, ghost)
+    tol = 1e-13 * float(np.abs(x * y).sum())
+    for f in (gpu.FMT_CSR, gpu.FMT_DIA):
+        yf, dot = H.spmv_dot(x, ghost, fmt=f)
+        np.testing.assert_array_equal(yf, y)
+        assert abs(dot - float(np.dot(x, y))) <= tol
+    H.destroy()

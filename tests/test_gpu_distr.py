@@ -8,7 +8,11 @@ host copy here.  Checks:
   * packed send buffers equal the receiving rank's ghost values,
   * y = A_loc x + halo blocks is bit-exact with the oracle (local CSR sum
     first, then the COO halo blocks in block order) and reassembles the
-    global product,
+    global product, through the plain distributed SpMV (CSR, JAD, COO) and
+    through the PCG's fused product kernel with the same ghost values (CSR
+    and DIA; the 7-point rows have 1 or 2 halo entries, which the kernel's
+    per-row record holds, the 27-point rows 4 to 15, which take the compact
+    lists),
   * a one-subdomain PCG with its six halo faces and zero ghost values is
     bit-identical to the plain local solve.
 """
@@ -78,6 +82,22 @@ def test_split_pack_and_distributed_spmv(gpu, name, nblocks):
             xn[rb["comm_perm"] - 1] = xs[q][rb["comm_perm"] - 1]
             yo = O.spmv_coo(rb["rows"], rb["cols"], rb["vals"], ref["row_offset"], rb["col_offset"], xn, yo)
         np.testing.assert_array_equal(y, yo)
+        # the solve's fused product (cel_gpu_spmv_dot, the PCG K1) with these
+        # ghost values: the same y bit for bit, x.y in its own order
+        dref = float(np.dot(xs[r], yo))
+        dtol = 1e-13 * float(np.abs(xs[r] * yo).sum())
+        yf, dot = mats[r].spmv_dot(xs[r], ghost)
+        np.testing.assert_array_equal(yf, yo)
+        assert abs(dot - dref) <= dtol
+        if name.startswith("s7"):  # (the small 27-point blocks have no DIA form)
+            mats[r].to_dia()
+            yf, dot = mats[r].spmv_dot(xs[r], ghost, fmt=gpu.FMT_DIA)
+            np.testing.assert_array_equal(yf, yo)
+            assert abs(dot - dref) <= dtol
+        # without the halo blocks: the local product
+        yl, _ = mats[r].spmv_dot(xs[r])
+        np.testing.assert_array_equal(
+            yl, O.spmv_csr(O.CSR(b0["xadj"], b0["cols"], b0["vals"], b0["col_offset"]), xs[r]))
         mats[r].to_jad()
         np.testing.assert_array_equal(mats[r].spmv_distr(xs[r], ghost, fmt=gpu.FMT_JAD), yo)
         # the local block as COO (mv_algorithm = MV_COO): every block's
