@@ -184,12 +184,19 @@ def test_batch_config4_workload(gpu):
     xs = lanes_of(out["x"], nrhs)
     lanes_b = lanes_of(bs, nrhs)
     owners = [m for m, k in zip(mats, nrhs) for _ in range(k)]
-    worst = 0.0
-    for M, b, x in zip(owners, lanes_b, xs):
-        rel = np.linalg.norm(b - M.spmv(x)) / np.linalg.norm(b)
-        worst = max(worst, rel)
-    print(f"worst true relative residual {worst:.3e}", flush=True)
-    assert worst <= 2e-9
+    worst, drift = 0.0, 0.0
+    for M, b, x, st in zip(owners, lanes_b, xs, out["lanes"]):
+        bn = np.linalg.norm(b)
+        true = np.linalg.norm(b - M.spmv(x))
+        # the stop rule: the recursive residual at most 1e-9 |b|
+        assert st["final_norm"] <= 1e-9 * bn * (1 + 1e-12), (st["final_norm"], bn)
+        worst = max(worst, true / bn)
+        drift = max(drift, abs(true - st["final_norm"]) / st["final_norm"])
+    print(f"worst true relative residual {worst:.3e}, recursive vs true {drift:.1e}", flush=True)
+    # the true residual follows the recursive one to rounding, so it meets the
+    # stated 1e-9 within that drift (measured worst true residual 9.989e-10)
+    assert drift <= 1e-2
+    assert worst <= 1e-9 * (1 + 1e-2)
     # spot checks against the single-system solver; the sum of their times
     # is the sequential alternative
     seq_ms = 0.0
