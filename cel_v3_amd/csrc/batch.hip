@@ -41,6 +41,8 @@
 namespace cel {
 
 constexpr int kMaxRhs = 64;
+// multi-rhs SpMV: row steps whose gathers are issued together
+constexpr int kBatchSteps = 4;
 constexpr int kUnroll = 4;  // elements per thread and step in the vector kernels
 constexpr int kBStage = 2048;  // staged entries per tile
 
@@ -60,6 +62,7 @@ struct BSys {
     int lane0;       // first lane of the system
     int b0, b1;      // current block range
     int ntiles;      // tiles of 256/kp rows
+    int period;      // multi-rhs: band period in 256-row chunks (0: none)
 };
 
 struct BBlk {
@@ -334,13 +337,30 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
         __shared__ int sx[kTileRows + 1];
         const int R = kBlock >> S.ksh;
         const int my = threadIdx.x >> S.ksh;
-        // chunks are dealt round-robin over the system's blocks (chunk c to
-        // block b0 + c mod nb), so the blocks sweep the rows together and
-        // rows a band apart are gathered close in time
+        // Chunk order.  A system on every block with a band period P (its
+        // bandwidth in chunks, P >= 8): XCD x takes the chunks at positions
+        // [P x / 8, P (x + 1) / 8) of every period, its blocks dealing them
+        // round-robin, so rows a band apart -- which gather the same x rows
+        // -- run on the same XCD and share its L2, and the XCD's blocks
+        // sweep its chunks together.  Otherwise chunks go round-robin over
+        // the system's blocks (chunk c to block b0 + c mod nb).
         const int nb = S.b1 - S.b0;
         const int first = b - S.b0;  // the table keeps b in [b0, b1)
         const int nchunks = (first >= 0 && first < nb) ? (S.ntiles + S.kp - 1) / S.kp : 0;
-        for (int c = first; c < nchunks; c += nb) {
+        const bool band = S.period >= kXcds && S.b0 == 0 && S.b1 == G && G % kXcds == 0;
+        const int gx = G / kXcds;                   // blocks per XCD (band order)
+        const int xcd = band ? b / gx : 0, jb = band ? b % gx : 0;
+        const int p0 = band ? S.period * xcd / kXcds : 0;
+        const int pc = band ? S.period * (xcd + 1) / kXcds - p0 : 1;  // positions of this XCD
+        for (int q = 0;; ++q) {
+            int c;
+            if (band) {
+                const long long rank = (long long)jb + (long long)q * gx;
+                c = (int)((rank / pc) * S.period + p0 + rank % pc);
+            } else {
+                c = first + q * nb;
+            }
+            if (c >= nchunks) break;  // uniform; chunk indices grow with q
             const int tc = c * S.kp;
             const int tn = min(tc + S.kp, S.ntiles);
             const int r0 = tc * R;
@@ -352,31 +372,101 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
             __syncthreads();  // the previous chunk's stage is consumed
             // the matrix is read once per product: non-temporal loads keep
             // it from evicting the gathered x lines from L2
-            for (int q = threadIdx.x; q <= r1 - r0; q += kBlock) sx[q] = ld_stream(S.xadj + r0 + q) - e0;
-            if (staged)
-                for (int k = threadIdx.x; k < cnt; k += kBlock) {
-                    sm.val[k] = ld_stream(S.vals + e0 + k);
-                    sm.col[k] = ld_stream(S.cols + e0 + k);
+            // (all of a thread's stage loads are issued before any is
+            // stored: a loop that stored each before loading the next paid
+            // one memory round trip per 256 entries)
+            {
+                constexpr int kSx = (kTileRows + 1 + kBlock - 1) / kBlock;
+                int xa[kSx];
+#pragma unroll
+                for (int i = 0; i < kSx; ++i) {
+                    const int q = threadIdx.x + i * kBlock;
+                    xa[i] = ld_stream(S.xadj + r0 + min(q, r1 - r0));
                 }
+#pragma unroll
+                for (int i = 0; i < kSx; ++i) {
+                    const int q = threadIdx.x + i * kBlock;
+                    if (q <= r1 - r0) sx[q] = xa[i] - e0;
+                }
+            }
+            if (staged && cnt > 0) {
+                constexpr int kPerT = kStageCap / kBlock;
+                double va[kPerT];
+                int ca[kPerT];
+#pragma unroll
+                for (int i = 0; i < kPerT; ++i) {
+                    const int k = min((int)threadIdx.x + i * kBlock, cnt - 1);
+                    va[i] = ld_stream(S.vals + e0 + k);
+                    ca[i] = ld_stream(S.cols + e0 + k);
+                }
+#pragma unroll
+                for (int i = 0; i < kPerT; ++i) {
+                    const int k = (int)threadIdx.x + i * kBlock;
+                    if (k < cnt) {
+                        sm.val[k] = va[i];
+                        sm.col[k] = ca[i];
+                    }
+                }
+            }
             __syncthreads();
             if (!on) continue;
+            if (staged) {
+                // kBatchSteps steps at a time: the gathers of all their rows
+                // (the first 8 entries of each, branch-free: an entry past
+                // the row reads x row 0 and is not added) and the rows' own
+                // p values are in flight together; the sums then run in
+                // entry order, rows longer than 8 continuing one entry at a
+                // time
+                for (int t = tc; t < tn; t += kBatchSteps) {
+                    int rw[kBatchSteps], lo[kBatchSteps], len[kBatchSteps];
+                    double xv[kBatchSteps][8], pr[kBatchSteps];
+#pragma unroll
+                    for (int u = 0; u < kBatchSteps; ++u) {
+                        rw[u] = (t + u) * R + my;
+                        const bool ok = t + u < tn && rw[u] < S.n;
+                        const int li = ok ? rw[u] - r0 : 0;
+                        lo[u] = ok ? sx[li] : 0;
+                        len[u] = ok ? sx[li + 1] - lo[u] : 0;
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            const int c = sm.col[min(lo[u] + i, kStageCap - 1)];
+                            xv[u][i] = ps[((size_t)(i < len[u] ? c : 0) << S.ksh) + j];
+                        }
+                        pr[u] = ps[((size_t)(ok ? rw[u] : 0) << S.ksh) + j];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kBatchSteps; ++u) {
+                        if (t + u < tn && rw[u] < S.n) {
+                            double sum = 0.0;
+#pragma unroll
+                            for (int i = 0; i < 8; ++i) {
+                                double prod = sm.val[min(lo[u] + i, kStageCap - 1)] * xv[u][i];
+                                const double tt = sum + prod;
+                                sum = i < len[u] ? tt : sum;
+                            }
+                            for (int k = lo[u] + 8; k < lo[u] + len[u]; ++k) {
+                                double prod = sm.val[k] * ps[((size_t)sm.col[k] << S.ksh) + j];
+                                sum = sum + prod;
+                            }
+                            const size_t e = ((size_t)rw[u] << S.ksh) + j;
+                            const double vi = 0.0 + sum;
+                            __builtin_nontemporal_store(vi, vs + e);  // read by the next kernel only
+                            double prod = vi * pr[u];
+                            acc = acc + prod;
+                        }
+                    }
+                }
+                continue;
+            }
             for (int t = tc; t < tn; ++t) {
                 const int row = t * R + my;
                 if (row < S.n) {
                     const int lo = sx[row - r0], hi = sx[row - r0 + 1];
-                    double sum = 0.0;
-                    if (staged) {
-#pragma unroll 4
-                        for (int k = lo; k < hi; ++k) {
-                            double prod = sm.val[k] * ps[((size_t)sm.col[k] << S.ksh) + j];
-                            sum = sum + prod;
-                        }
-                    } else {
-                        for (int k = lo; k < hi; ++k) {
-                            double prod = ld_stream(S.vals + e0 + k) *
-                                          ps[((size_t)ld_stream(S.cols + e0 + k) << S.ksh) + j];
-                            sum = sum + prod;
-                        }
+                    double sum = 0.0;  // a chunk too large for the stage: entries from HBM
+                    for (int k = lo; k < hi; ++k) {
+                        double prod = ld_stream(S.vals + e0 + k) *
+                                      ps[((size_t)ld_stream(S.cols + e0 + k) << S.ksh) + j];
+                        sum = sum + prod;
                     }
                     const size_t e = ((size_t)row << S.ksh) + j;
                     const double vi = 0.0 + sum;
@@ -895,6 +985,45 @@ struct cel_gpu_batch {
     Batch b;
 };
 
+// Bandwidth max |col - row| of a square block (0-based rows and columns):
+// per-block maxima, the host takes the largest.
+__global__ __launch_bounds__(kBlock) void k_bandwidth(int n, const int* __restrict__ xadj,
+                                                      const int* __restrict__ cols,
+                                                      int* __restrict__ out)
+{
+    __shared__ int wm[kWaves];
+    int m = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        for (int k = xadj[i]; k < xadj[i + 1]; ++k) m = max(m, abs(cols[k] - (int)i));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int b = 0;
+        for (int w = 0; w < kWaves; ++w) b = max(b, wm[w]);
+        out[blockIdx.x] = b;
+    }
+}
+
+static int block_bandwidth(const Mat& A, int& bw)
+{
+    bw = 0;
+    if (A.n == 0) return CEL_OK;
+    constexpr int kG = 256;
+    DevBuf<int> part;
+    CEL_TRY(part.alloc(kG));
+    hipStream_t s = rt().stream;
+    hipLaunchKernelGGL(k_bandwidth, dim3(kG), dim3(kBlock), 0, s, A.n, A.xadj.p, A.cols.p, part.p);
+    CEL_HIP(hipGetLastError());
+    std::vector<int> h(kG);
+    CEL_HIP(hipMemcpyAsync(h.data(), part.p, sizeof(int) * kG, hipMemcpyDeviceToHost, s));
+    CEL_HIP(hipStreamSynchronize(s));
+    for (int v : h) bw = std::max(bw, v);
+    return CEL_OK;
+}
+
 extern "C" int cel_gpu_batch_create(cel_gpu_batch** out, int nsys, cel_gpu_mat* const* mats,
                                     const int* nrhs)
 {
@@ -935,6 +1064,11 @@ extern "C" int cel_gpu_batch_create(cel_gpu_batch** out, int nsys, cel_gpu_mat* 
         S.ksh = ksh;
         S.lane0 = lane;
         S.ntiles = (int)(((long long)A->n + (kBlock >> ksh) - 1) / (kBlock >> ksh));
+        if (kp > 1) {  // band period of the multi-rhs chunk order, in 256-row chunks
+            int bw = 0;
+            CEL_TRY(block_bandwidth(*A, bw));
+            S.period = (bw + kTileRows / 2) / kTileRows;
+        }
         B.h_sys.push_back(S);
         off += (long long)A->n * kp;
         lane += kp;
