@@ -198,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_bresidual(const BSys* __restrict__ s
 {
     __shared__ BStage sm;
     __shared__ double red[4 * 64];
-    const int b = logical_block();
+    const int b = blockIdx.x;  // hardware order: a block range has equal shares of every XCD
     const BBlk blk = tbl[b];
     if (blk.sys < 0) return;
     const BSys S = sys[blk.sys];
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
     __shared__ double s1[kMaxRhs], s2[kMaxRhs], s3[kMaxRhs];
     __shared__ BCtrl lc[kMaxRhs];
     __shared__ int any;
-    const int b = logical_block();
+    const int b = blockIdx.x;  // hardware order: a block range has equal shares of every XCD
     const BBlk blk = tbl[b];
     if (blk.sys < 0) return;
     const BSys S = sys[blk.sys];
@@ -337,19 +337,24 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
         __shared__ int sx[kTileRows + 1];
         const int R = kBlock >> S.ksh;
         const int my = threadIdx.x >> S.ksh;
-        // Chunk order.  A system on every block with a band period P (its
-        // bandwidth in chunks, P >= 8): XCD x takes the chunks at positions
-        // [P x / 8, P (x + 1) / 8) of every period, its blocks dealing them
-        // round-robin, so rows a band apart -- which gather the same x rows
-        // -- run on the same XCD and share its L2, and the XCD's blocks
-        // sweep its chunks together.  Otherwise chunks go round-robin over
-        // the system's blocks (chunk c to block b0 + c mod nb).
+        // Chunk order.  A system with a band period P (its bandwidth in
+        // chunks, P >= 8) on at least 8 blocks: XCD x takes the chunks at
+        // positions [P x / 8, P (x + 1) / 8) of every period, its blocks of
+        // the system dealing them round-robin, so rows a band apart --
+        // which gather the same x rows -- run on the same XCD and share its
+        // L2, and the XCD's blocks sweep its chunks together.  Otherwise
+        // chunks go round-robin over the system's blocks (chunk c to block
+        // b0 + c mod nb).
         const int nb = S.b1 - S.b0;
         const int first = b - S.b0;  // the table keeps b in [b0, b1)
         const int nchunks = (first >= 0 && first < nb) ? (S.ntiles + S.kp - 1) / S.kp : 0;
-        const bool band = S.period >= kXcds && S.b0 == 0 && S.b1 == G && G % kXcds == 0;
-        const int gx = G / kXcds;                   // blocks per XCD (band order)
-        const int xcd = band ? b / gx : 0, jb = band ? b % gx : 0;
+        // (hardware block b runs on XCD b mod 8; the system's blocks on XCD x
+        // are b0 <= b < b1 with b = x mod 8)
+        const bool band = S.period >= kXcds && nb >= kXcds;
+        const int xcd = b % kXcds;
+        const int bx0 = S.b0 + ((xcd - S.b0 % kXcds) + kXcds) % kXcds;  // first of them
+        const int gx = band ? (S.b1 - 1 - bx0) / kXcds + 1 : 1;       // their number
+        const int jb = band ? (b - bx0) / kXcds : 0;
         const int p0 = band ? S.period * xcd / kXcds : 0;
         const int pc = band ? S.period * (xcd + 1) / kXcds - p0 : 1;  // positions of this XCD
         for (int q = 0;; ++q) {
@@ -497,7 +502,7 @@ __global__ __launch_bounds__(kBlock) void k_bupdate_r(const BSys* __restrict__ s
     __shared__ double s1[kMaxRhs];
     __shared__ BCtrl lc[kMaxRhs];
     __shared__ int any;
-    const int b = logical_block();
+    const int b = blockIdx.x;  // hardware order: a block range has equal shares of every XCD
     const BBlk blk = tbl[b];
     if (blk.sys < 0) return;
     const BSys S = sys[blk.sys];
@@ -594,7 +599,7 @@ __global__ __launch_bounds__(kBlock) void k_bupdate_xp(const BSys* __restrict__ 
     __shared__ double s1[kMaxRhs], s2[kMaxRhs], sb[kMaxRhs];
     __shared__ BCtrl lc[kMaxRhs];
     __shared__ int any;
-    const int b = logical_block();
+    const int b = blockIdx.x;  // hardware order: a block range has equal shares of every XCD
     const BBlk blk = tbl[b];
     if (blk.sys < 0) return;
     const BSys S = sys[blk.sys];
