@@ -199,6 +199,31 @@ def batch_bench(cel, steps):
             "speedup": seq_ms / float(np.median(times)), "tune_s": tune_s}
 
 
+def batch16_bench(cel, steps):
+    """The batch's multi-right-hand-side system alone: 128^3 Poisson with 16
+    right-hand sides ~ U[-1,1] (numpy default_rng(3)), FP64 to 1e-9
+    relative, one tuned batch; time per iteration and the algorithmic bytes
+    of its SpMV (CSR matrix once for the 16 lanes, x and y as 16-lane rows)."""
+    P = cel.Matrix.poisson7(128, 128, 128)
+    P.set_jacobi()
+    bs = [np.random.default_rng(3).uniform(-1.0, 1.0, (16, 128 ** 3))]
+    batch = cel.Batch([P], [16])
+    info = batch.tune(bs, iters=32)
+    kw = dict(tol=1e-9, tol_mode=cel.TOL_REL_B, alpha_stop=0, iter_max=20000)
+    times, iters = [], 0
+    for k in range(max(1, steps)):
+        out = batch.solve(bs, **kw)
+        times.append(out["total"]["solve_ms"])
+        iters = out["total"]["iters"]
+        log(f"128^3 x 16 solve {k + 1}: {times[-1]:.1f} ms, {iters} iterations, "
+            f"{times[-1] / iters * 1e3:.1f} us per iteration")
+    n = P.n
+    spmv_bytes = 12 * P.nnz + 4 * (n + 1) + 2 * 16 * 8 * n
+    return {"workload": "poisson7 128^3 x 16 rhs, tol 1e-9", "tuned": info, "solve_ms": times,
+            "iters": iters, "us_per_iter": float(np.median(times)) / iters * 1e3,
+            "spmv_algorithmic_bytes": spmv_bytes}
+
+
 def gmres_bench(cel, N, steps, iter_cap):
     """Jacobi-GMRES with the element solver's settings (m = 3, eps 1e-8 on
     the true residual norm, x0 = 1) on the N^3 7-point block, b ~ U[0,1)
@@ -404,8 +429,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true", help="do not time the CPU baseline")
     ap.add_argument("--skip-config3", action="store_true", help="do not run the 384^3 workload")
     ap.add_argument("--only", default="all",
-                    choices=["all", "fp64", "mixed", "spmv", "profile", "batch", "sort", "gmres",
-                             "config3", "bandwidth"],
+                    choices=["all", "fp64", "mixed", "spmv", "profile", "batch", "batch16", "sort",
+                             "gmres", "config3", "bandwidth"],
                     help="run one part only; 'profile' = one streaming copy of known size "
                          "(counter calibration) and FP64, FP64-with-halo and mixed solves "
                          "capped at --iters iterations in every form, for hardware-counter "
@@ -414,7 +439,8 @@ def main():
                          "COO ordering of the shuffled --size^3 matrix in HBM; 'gmres' = "
                          "Jacobi-GMRES(3) with the element solver's settings on the 128^3 block "
                          "(--iters > 0 caps its outer iterations); 'config3' = the 384^3 "
-                         "precision workload; 'bandwidth' = HBM read rate of 1-8 arrays read "
+                         "precision workload; 'batch16' = the batch's 128^3 x 16 system alone "
+                         "(the multi-rhs SpMV); 'bandwidth' = HBM read rate of 1-8 arrays read "
                          "together and of a copy (the measured ceiling beside the 8 TB/s peak)")
     ap.add_argument("--iters", type=int, default=None,
                     help="iteration cap of --only profile (default 20) and --only gmres (default none)")
@@ -438,6 +464,10 @@ def main():
     if args.only == "gmres":
         cel.init(0)
         print(json.dumps(gmres_bench(cel, 128, args.steps, args.iters or 0)))
+        return
+    if args.only == "batch16":
+        cel.init(0)
+        print(json.dumps(batch16_bench(cel, args.steps)))
         return
     if args.only == "bandwidth":
         cel.init(0)
