@@ -7,12 +7,13 @@
 // (alpha, sigma, norm, iteration count, stop decision), and each lane runs
 // exactly the reference's op sequence.
 //
-// Layout.  The vectors of all systems are concatenated; system s owns
-// n_s * kp_s doubles, stored row-major [row][kp_s] with kp_s = the rhs
-// count rounded up to a power of two (padding lanes get b = 0 and stop at
-// once).  A 256-thread block therefore covers 256/kp rows x kp lanes: in
-// the SpMV the kp lanes of one row read x[col][0..kp) as one contiguous
-// segment and the matrix entries of a row are read once for all kp lanes.
+// Layout.  The vectors of all systems are concatenated, each starting at
+// a multiple of 64 doubles; system s owns n_s * kp_s doubles, stored
+// row-major [row][kp_s] with kp_s = the rhs count rounded up to a power of
+// two (padding lanes get b = 0 and stop at once).  In the SpMV the kp lanes
+// of one row read x[col][0..kp) as one contiguous segment (kp/2 threads,
+// two lanes each) and the matrix entries of a row are read once for all kp
+// lanes; the vector kernels cover 256 elements per block and step.
 //
 // Work table.  Every block of the persistent grid is assigned to one
 // system (a contiguous block range per system, sized by the system's work)
@@ -41,10 +42,16 @@
 namespace cel {
 
 constexpr int kMaxRhs = 64;
-// multi-rhs SpMV: row steps whose gathers are issued together
-constexpr int kBatchSteps = 4;
+// multi-rhs SpMV: row steps whose gathers are issued together (one: 80
+// VGPRs, four waves per SIMD; two or four steps measured slower)
+constexpr int kPairSteps = 1;
+typedef double Dbl2 __attribute__((ext_vector_type(2)));
 constexpr int kUnroll = 4;  // elements per thread and step in the vector kernels
 constexpr int kBStage = 2048;  // staged entries per tile
+// every system's vectors start at a multiple of 64 doubles (512 bytes): the
+// kp-lane rows of a multi-rhs system then never straddle a 128-byte line
+// and the lane pairs of its SpMV are 16-byte aligned
+constexpr int kVecAlign = 64;
 
 struct BCtrl {
     double alpha, sigma, norm2, eps, norm0, bnorm, pad0, pad1;
@@ -93,6 +100,33 @@ __device__ __forceinline__ double lane_block_sum(double v, int kp, double* smem 
         if (off >= kp) v = v + __shfl_xor(v, off, 64);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane < kp) smem[wave * 64 + lane] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x < kp) {
+        for (int w = 0; w < kWaves; ++w) s = s + smem[w * 64 + threadIdx.x];
+    }
+    __syncthreads();
+    return s;
+}
+
+// The same for threads that hold two adjacent lanes each (kp >= 2): thread
+// tid holds lanes 2 (tid & (kp/2 - 1)) and that + 1.  Valid in threads
+// tid < kp, which get lane tid.
+__device__ __forceinline__ double lane_pair_block_sum(double v0, double v1, int kp,
+                                                      double* smem /* 4*64 */)
+{
+    const int h = kp >> 1;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        if (off >= h) {
+            v0 = v0 + __shfl_xor(v0, off, 64);
+            v1 = v1 + __shfl_xor(v1, off, 64);
+        }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane < h) {
+        smem[wave * 64 + 2 * lane] = v0;
+        smem[wave * 64 + 2 * lane + 1] = v1;
+    }
     __syncthreads();
     double s = 0.0;
     if (threadIdx.x < kp) {
@@ -291,8 +325,6 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
         write_lanes(cout, S, b, lc);
         return;
     }
-    const int j = threadIdx.x & (S.kp - 1);
-    const bool on = lc[j].run != 0;
     const double* ps = p + S.voff;
     double* vs = v + S.voff;
     double acc = 0.0;
@@ -326,17 +358,26 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
             }
         }
     } else {
-        // kp right-hand sides: the kp lanes of a row gather x[col][j]
-        // contiguously.  The block's steps (R = 256/kp rows each) go in
-        // chunks of kp steps = 256 rows whose row pointers and entries are
-        // staged in LDS with coalesced loads first, so a row's only global
-        // loads are its gathers (a row read straight from HBM waits on its
-        // row bounds, then its entries, then the gathers).  Rows are summed
-        // in entry order and added to the partial in step order; a chunk
-        // whose entries do not fit the stage reads them from HBM.
+        // kp right-hand sides, two adjacent lanes per thread: the kp/2
+        // threads of a row gather x[col][0..kp) contiguously, 16 bytes each
+        // (one load instruction serves two lanes).  The block's steps go in
+        // chunks of 256 rows whose row pointers and entries are staged in
+        // LDS with coalesced loads first, so a row's only global loads are
+        // its gathers (a row read straight from HBM waits on its row bounds,
+        // then its entries, then the gathers).  Every lane sums its row in
+        // entry order; the rows of a thread are added to its partials in
+        // row order.  A chunk whose entries do not fit the stage reads them
+        // from HBM.
         __shared__ int sx[kTileRows + 1];
-        const int R = kBlock >> S.ksh;
-        const int my = threadIdx.x >> S.ksh;
+        const int kh = S.ksh - 1;                              // kp/2 = 2^kh threads per row
+        const int R2 = kBlock >> kh;                           // rows per step
+        const int my = threadIdx.x >> kh;
+        const int jl = 2 * (threadIdx.x & ((S.kp >> 1) - 1));  // the thread's first lane
+        const bool on2 = lc[jl].run != 0 || lc[jl + 1].run != 0;
+        // (voff and kp are even, so the pairs are 16-byte aligned)
+        const double* pl = ps + jl;
+        double* vl = vs + jl;
+        double acc1 = 0.0;
         // Chunk order.  A system with a band period P (its bandwidth in
         // chunks, P >= 8) on at least 8 blocks: XCD x takes the chunks at
         // positions [P x / 8, P (x + 1) / 8) of every period, its blocks of
@@ -366,11 +407,10 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
                 c = first + q * nb;
             }
             if (c >= nchunks) break;  // uniform; chunk indices grow with q
-            const int tc = c * S.kp;
-            const int tn = min(tc + S.kp, S.ntiles);
-            const int r0 = tc * R;
-            if (r0 >= S.n) break;  // uniform: steps past the last row hold no rows
-            const int r1 = min(tn * R, S.n);
+            // chunk c = tiles [c kp, c kp + kp) of 256/kp rows = rows [r0, r1)
+            const int r0 = c * kTileRows;
+            if (r0 >= S.n) break;  // uniform
+            const int r1 = min(r0 + kTileRows, S.n);
             const int e0 = S.xadj[r0];
             const int cnt = S.xadj[r1] - e0;
             const bool staged = cnt <= kStageCap;  // uniform across the block
@@ -414,73 +454,97 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
                 }
             }
             __syncthreads();
-            if (!on) continue;
+            if (!on2) continue;
+            const int nst = (r1 - r0 + R2 - 1) / R2;  // steps of this chunk
             if (staged) {
-                // kBatchSteps steps at a time: the gathers of all their rows
+                // kPairSteps steps at a time: the gathers of all their rows
                 // (the first 8 entries of each, branch-free: an entry past
                 // the row reads x row 0 and is not added) and the rows' own
                 // p values are in flight together; the sums then run in
                 // entry order, rows longer than 8 continuing one entry at a
                 // time
-                for (int t = tc; t < tn; t += kBatchSteps) {
-                    int rw[kBatchSteps], lo[kBatchSteps], len[kBatchSteps];
-                    double xv[kBatchSteps][8], pr[kBatchSteps];
+                for (int s0 = 0; s0 < nst; s0 += kPairSteps) {
+                    int rw[kPairSteps], lo[kPairSteps], len[kPairSteps];
+                    Dbl2 xv[kPairSteps][8], pr[kPairSteps];
 #pragma unroll
-                    for (int u = 0; u < kBatchSteps; ++u) {
-                        rw[u] = (t + u) * R + my;
-                        const bool ok = t + u < tn && rw[u] < S.n;
+                    for (int u = 0; u < kPairSteps; ++u) {
+                        rw[u] = r0 + (s0 + u) * R2 + my;
+                        const bool ok = s0 + u < nst && rw[u] < r1;
                         const int li = ok ? rw[u] - r0 : 0;
                         lo[u] = ok ? sx[li] : 0;
                         len[u] = ok ? sx[li + 1] - lo[u] : 0;
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
                             const int c = sm.col[min(lo[u] + i, kStageCap - 1)];
-                            xv[u][i] = ps[((size_t)(i < len[u] ? c : 0) << S.ksh) + j];
+                            xv[u][i] = *(const Dbl2*)(pl + ((size_t)(i < len[u] ? c : 0) << S.ksh));
                         }
-                        pr[u] = ps[((size_t)(ok ? rw[u] : 0) << S.ksh) + j];
+                        pr[u] = *(const Dbl2*)(pl + ((size_t)(ok ? rw[u] : 0) << S.ksh));
                     }
 #pragma unroll
-                    for (int u = 0; u < kBatchSteps; ++u) {
-                        if (t + u < tn && rw[u] < S.n) {
-                            double sum = 0.0;
+                    for (int u = 0; u < kPairSteps; ++u) {
+                        if (s0 + u < nst && rw[u] < r1) {
+                            double sum0 = 0.0, sum1 = 0.0;
 #pragma unroll
                             for (int i = 0; i < 8; ++i) {
-                                double prod = sm.val[min(lo[u] + i, kStageCap - 1)] * xv[u][i];
-                                const double tt = sum + prod;
-                                sum = i < len[u] ? tt : sum;
+                                const double a = sm.val[min(lo[u] + i, kStageCap - 1)];
+                                double prod0 = a * xv[u][i].x;
+                                double prod1 = a * xv[u][i].y;
+                                const double t0 = sum0 + prod0, t1 = sum1 + prod1;
+                                sum0 = i < len[u] ? t0 : sum0;
+                                sum1 = i < len[u] ? t1 : sum1;
                             }
                             for (int k = lo[u] + 8; k < lo[u] + len[u]; ++k) {
-                                double prod = sm.val[k] * ps[((size_t)sm.col[k] << S.ksh) + j];
-                                sum = sum + prod;
+                                const double a = sm.val[k];
+                                const Dbl2 xk = *(const Dbl2*)(pl + ((size_t)sm.col[k] << S.ksh));
+                                double prod0 = a * xk.x;
+                                double prod1 = a * xk.y;
+                                sum0 = sum0 + prod0;
+                                sum1 = sum1 + prod1;
                             }
-                            const size_t e = ((size_t)rw[u] << S.ksh) + j;
-                            const double vi = 0.0 + sum;
-                            __builtin_nontemporal_store(vi, vs + e);  // read by the next kernel only
-                            double prod = vi * pr[u];
-                            acc = acc + prod;
+                            Dbl2 vi;
+                            vi.x = 0.0 + sum0;
+                            vi.y = 0.0 + sum1;
+                            // read by the next kernel only
+                            __builtin_nontemporal_store(vi, (Dbl2*)(vl + ((size_t)rw[u] << S.ksh)));
+                            double d0 = vi.x * pr[u].x;
+                            acc = acc + d0;
+                            double d1 = vi.y * pr[u].y;
+                            acc1 = acc1 + d1;
                         }
                     }
                 }
                 continue;
             }
-            for (int t = tc; t < tn; ++t) {
-                const int row = t * R + my;
-                if (row < S.n) {
+            for (int st = 0; st < nst; ++st) {
+                const int row = r0 + st * R2 + my;
+                if (row < r1) {
                     const int lo = sx[row - r0], hi = sx[row - r0 + 1];
-                    double sum = 0.0;  // a chunk too large for the stage: entries from HBM
+                    double sum0 = 0.0, sum1 = 0.0;  // a chunk too large for the stage: entries from HBM
                     for (int k = lo; k < hi; ++k) {
-                        double prod = ld_stream(S.vals + e0 + k) *
-                                      ps[((size_t)ld_stream(S.cols + e0 + k) << S.ksh) + j];
-                        sum = sum + prod;
+                        const double a = ld_stream(S.vals + e0 + k);
+                        const Dbl2 xk =
+                            *(const Dbl2*)(pl + ((size_t)ld_stream(S.cols + e0 + k) << S.ksh));
+                        double prod0 = a * xk.x;
+                        double prod1 = a * xk.y;
+                        sum0 = sum0 + prod0;
+                        sum1 = sum1 + prod1;
                     }
-                    const size_t e = ((size_t)row << S.ksh) + j;
-                    const double vi = 0.0 + sum;
-                    __builtin_nontemporal_store(vi, vs + e);  // read by the next kernel only
-                    double prod = vi * ps[e];
-                    acc = acc + prod;
+                    Dbl2 vi;
+                    vi.x = 0.0 + sum0;
+                    vi.y = 0.0 + sum1;
+                    __builtin_nontemporal_store(vi, (Dbl2*)(vl + ((size_t)row << S.ksh)));
+                    const Dbl2 pv = *(const Dbl2*)(pl + ((size_t)row << S.ksh));
+                    double d0 = vi.x * pv.x;
+                    acc = acc + d0;
+                    double d1 = vi.y * pv.y;
+                    acc1 = acc1 + d1;
                 }
             }
         }
+        const double bs = lane_pair_block_sum(acc, acc1, S.kp, red);
+        if (threadIdx.x < S.kp) part[(size_t)b * kpmax + threadIdx.x] = bs;
+        write_lanes(cout, S, b, lc);
+        return;
     }
     const double bs = lane_block_sum(acc, S.kp, red);
     if (threadIdx.x < S.kp) part[(size_t)b * kpmax + threadIdx.x] = bs;
@@ -714,7 +778,8 @@ struct Batch {
     std::vector<Mat*> mats;
     std::vector<int> k, kp, ksh, lane0;
     std::vector<long long> voff;
-    long long total = 0;  // vector elements
+    long long total = 0;  // vector elements, each system's padded to kVecAlign
+    long long elems = 0;  // vector elements of the systems
     int lanes = 0;        // padded lanes
     int kpmax = 1;
     DevBuf<BSys> d_sys;
@@ -1075,7 +1140,8 @@ extern "C" int cel_gpu_batch_create(cel_gpu_batch** out, int nsys, cel_gpu_mat* 
             S.period = (bw + kTileRows / 2) / kTileRows;
         }
         B.h_sys.push_back(S);
-        off += (long long)A->n * kp;
+        off += ((long long)A->n * kp + kVecAlign - 1) / kVecAlign * kVecAlign;
+        B.elems += (long long)A->n * kp;
         lane += kp;
         B.kpmax = std::max(B.kpmax, kp);
     }
@@ -1105,7 +1171,7 @@ extern "C" int cel_gpu_batch_info(const cel_gpu_batch* h, int* lanes, long long*
     int nl = 0;
     for (int k : B.k) nl += k;
     if (lanes) *lanes = nl;
-    if (elements) *elements = B.total;
+    if (elements) *elements = B.elems;
     if (grid) *grid = B.tune.grid;
     if (chunk) *chunk = B.tune.chunk;
     if (weight) *weight = B.tune.weight;

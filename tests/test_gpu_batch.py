@@ -222,6 +222,45 @@ def test_batch_rejects_mixed_precision(gpu):
         batch.solve([np.ones(M.n)], precision=gpu.PREC_MIXED)
 
 
+def test_batch_two_and_sixty_four_right_hand_sides(gpu):
+    """The smallest and the largest lane blocks of the multi-rhs SpMV (one
+    thread per row holding both lanes; 32 threads per row), behind a one-rhs
+    system of odd size so that the next system's vectors start after
+    padding; every lane is checked against the oracle."""
+    rng = np.random.default_rng(23)
+    shapes = [((3, 3, 3), 1), ((6, 5, 4), 2), ((5, 5, 5), 64), ((7, 3, 5), 33)]
+    mats, bs, nrhs, hosts = [], [], [], []
+    for sh, k in shapes:
+        M = gpu.Matrix.poisson7(*sh)
+        M.set_jacobi()
+        n = int(np.prod(sh))
+        B = rng.uniform(-1.0, 1.0, (k, n))
+        B *= S / np.linalg.norm(B, axis=1, keepdims=True)
+        mats.append(M)
+        bs.append(B[0] if k == 1 else B)
+        nrhs.append(k)
+        hosts.append(O.gen_poisson7(*sh))
+    out = gpu.Batch(mats, nrhs).solve(bs, tol=1e-9, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO,
+                                      alpha_stop=1)
+    assert out["rc"] == 0, out["total"]
+    xs = lanes_of(out["x"], nrhs)
+    lane = 0
+    for A, b, k in zip(hosts, bs, nrhs):
+        jac = O.set_jacobi(O.csr_diagonal(A))
+        for j in range(k):
+            bj = b if k == 1 else b[j]
+            ref = O.pcg_jacobi(A, jac, bj, x0_mode=O.X0_ZERO, eps=1e-9 * np.linalg.norm(bj),
+                               iter_max=10000, workers=1)
+            st = out["lanes"][lane]
+            assert st["status"] == 0
+            assert abs(st["iters"] - ref["iters"]) <= 1, (lane, st["iters"], ref["iters"])
+            if st["iters"] == ref["iters"]:
+                np.testing.assert_allclose(xs[lane], ref["x"], rtol=0,
+                                           atol=1e-10 * np.abs(ref["x"]).max())
+            lane += 1
+    assert lane == 1 + 2 + 64 + 33
+
+
 def test_batch_rows_beyond_the_lds_stage(gpu):
     """256-row chunks with more than 2048 entries (27-point rows) read their
     entries from HBM instead of the LDS stage; a block-diagonal matrix of a
