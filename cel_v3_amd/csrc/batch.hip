@@ -138,6 +138,8 @@ __device__ __forceinline__ double lane_pair_block_sum(double v0, double v1, int 
 
 // Sum over blocks [b0, b1) of part[b * kpmax + j] for every lane j < kp,
 // in a fixed order; result in out[j] (LDS), visible to the whole block.
+// Plain loads: every block of the system reads the same partials, which
+// after the first read on an XCD sit in its L2.
 __device__ __forceinline__ void lane_partials(const double* __restrict__ part, int kpmax, int b0,
                                               int b1, int kp, int ksh, double* out, double* smem)
 {
@@ -145,7 +147,7 @@ __device__ __forceinline__ void lane_partials(const double* __restrict__ part, i
     const int step = kBlock >> ksh;
     double acc = 0.0;
     for (int b = b0 + (threadIdx.x >> ksh); b < b1; b += step)
-        acc = acc + __builtin_nontemporal_load(part + (size_t)b * kpmax + j);
+        acc = acc + part[(size_t)b * kpmax + j];
     const double s = lane_block_sum(acc, kp, smem);
     if (threadIdx.x < kp) out[threadIdx.x] = s;
     __syncthreads();

@@ -84,10 +84,12 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* smem /*
 
 // Sum of G partials in logical-block order by the whole block: thread t
 // sums t, t+256, ... sequentially, then block_sum.  Valid in thread 0.
+// Every block of the consuming kernel reads the same partials, so they are
+// plain (cached) loads: after the first block on an XCD the rest hit its L2.
 __device__ __forceinline__ double sum_partials(const double* partials, int G, double* smem)
 {
     double s = 0.0;
-    for (int i = threadIdx.x; i < G; i += kBlock) s = s + __builtin_nontemporal_load(partials + i);
+    for (int i = threadIdx.x; i < G; i += kBlock) s = s + partials[i];
     return block_sum(s, smem);
 }
 

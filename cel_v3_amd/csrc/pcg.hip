@@ -107,8 +107,8 @@ __device__ __forceinline__ void all_sum2(const double* __restrict__ pa, const do
 {
     double sa = 0.0, sb = 0.0;
     for (int i = threadIdx.x; i < G; i += kBlock) {
-        sa = sa + __builtin_nontemporal_load(pa + i);
-        sb = sb + __builtin_nontemporal_load(pb + i);
+        sa = sa + pa[i];
+        sb = sb + pb[i];
     }
     block_sum2(sa, sb, red);
     if (threadIdx.x == 0) {
