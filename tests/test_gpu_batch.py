@@ -222,13 +222,15 @@ def test_batch_rejects_mixed_precision(gpu):
         batch.solve([np.ones(M.n)], precision=gpu.PREC_MIXED)
 
 
-def test_batch_two_and_sixty_four_right_hand_sides(gpu):
-    """The smallest and the largest lane blocks of the multi-rhs SpMV (one
-    thread per row holding both lanes; 32 threads per row), behind a one-rhs
-    system of odd size so that the next system's vectors start after
-    padding; every lane is checked against the oracle."""
+def test_batch_every_lane_block_width(gpu):
+    """Every lane-block width of the multi-rhs SpMV, kp = 2 (one thread per
+    row holding both lanes) to 64 (32 threads per row), with padding lanes
+    (20 -> 32, 33 -> 64), behind a one-rhs system of odd size so that the
+    next system's vectors start after padding; every lane is checked
+    against the oracle."""
     rng = np.random.default_rng(23)
-    shapes = [((3, 3, 3), 1), ((6, 5, 4), 2), ((5, 5, 5), 64), ((7, 3, 5), 33)]
+    shapes = [((3, 3, 3), 1), ((6, 5, 4), 2), ((5, 5, 5), 64), ((7, 3, 5), 33),
+              ((4, 6, 5), 8), ((5, 4, 3), 20), ((3, 4, 5), 4)]
     mats, bs, nrhs, hosts = [], [], [], []
     for sh, k in shapes:
         M = gpu.Matrix.poisson7(*sh)
@@ -258,7 +260,7 @@ def test_batch_two_and_sixty_four_right_hand_sides(gpu):
                 np.testing.assert_allclose(xs[lane], ref["x"], rtol=0,
                                            atol=1e-10 * np.abs(ref["x"]).max())
             lane += 1
-    assert lane == 1 + 2 + 64 + 33
+    assert lane == 1 + 2 + 64 + 33 + 8 + 20 + 4
 
 
 def test_batch_rows_beyond_the_lds_stage(gpu):
