@@ -82,6 +82,7 @@ extern "C" int cel_gpu_finalize(void)
     Runtime& R = rt();
     if (!R.ready) return CEL_OK;
     (void)hipStreamSynchronize(R.stream);
+    coo_release_workspace();
     (void)hipStreamDestroy(R.stream);
     R.stream = nullptr;
     R.ready = false;

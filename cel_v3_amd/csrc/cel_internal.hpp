@@ -66,6 +66,12 @@ struct DevBuf {
         n = count;
         return CEL_OK;
     }
+    // at least `count` elements, keeping a larger allocation (a workspace
+    // reused from call to call)
+    int reserve(size_t count) {
+        if (p && n >= count) return CEL_OK;
+        return alloc(count);
+    }
     void release() {
         if (p) (void)hipFree(p);
         p = nullptr;
@@ -245,6 +251,8 @@ int launch_scale_rows(Mat& A);
 int launch_scale_vec(const Mat& A, double* b);
 int launch_to_f32(const double* src, float* dst, size_t n);
 int launch_coo_to_csr(int nrows, int nnz, const int* rows0, int* xadj);  // rows 0-based local
+// Frees the workspace the COO sort and row-offset routines keep between calls.
+void coo_release_workspace();
 int jad_convert_device(Mat& A, int mode);
 // Builds A.jad_pcols (once per conversion).
 int jad_prepare_permuted(Mat& A);

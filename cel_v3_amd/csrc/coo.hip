@@ -371,13 +371,41 @@ k_sub_int(long long n, const int* a, int v, int* out)  // may run in place
     CEL_GRID_STRIDE(i, n) out[i] = a[i] - v;
 }
 
+// The sort's device workspace, kept from call to call on the device that
+// made it and grown when a larger sort needs more (a 117M-entry sort holds
+// about 6 GB); allocating and freeing it per call cost about 5 ms of a
+// 14 ms sort.  cel_gpu_finalize frees it.  Calls that overlap from several
+// host threads share it and race.
 struct SortBufs {
     DevBuf<uint32_t> rk0, rk1;
     DevBuf<Pay> pay0, pay1;
     DevBuf<int> a, b, c, starts, idx, perm, lists;
     DevBuf<uint64_t> k0, k1;
     DevBuf<unsigned char> temp;
+    DevBuf<int> rows, cols, xadj, flag, cnt;  // staging and outputs of the entry points
+    DevBuf<double> vals;
+    int device = -1;
+    void release()
+    {
+        rk0.release(); rk1.release(); pay0.release(); pay1.release();
+        a.release(); b.release(); c.release(); starts.release(); idx.release(); perm.release();
+        lists.release(); k0.release(); k1.release(); temp.release();
+        rows.release(); cols.release(); xadj.release(); flag.release(); cnt.release();
+        vals.release();
+    }
 };
+
+// Never destroyed: a static destructor could run after the HIP runtime has
+// gone at process exit.
+SortBufs& sort_cache()
+{
+    static SortBufs* S = new SortBufs;
+    if (S->device != rt().device) {
+        S->release();
+        S->device = rt().device;
+    }
+    return *S;
+}
 
 int temp_fit(SortBufs& S, size_t bytes)
 {
@@ -405,11 +433,11 @@ int sort_device(SortBufs& S, int nrows, int ncols, long long nnz, int rbase, int
     const int g = grid_for(nnz);
     const int n = (int)nnz;
     const int rb = bits_for(nrows > 0 ? (unsigned long long)(nrows - 1) : 0ULL);
-    CEL_TRY(S.rk0.alloc(nnz));
-    CEL_TRY(S.rk1.alloc(nnz));
-    CEL_TRY(S.pay0.alloc(nnz));
-    CEL_TRY(S.pay1.alloc(nnz));
-    CEL_TRY(S.lists.alloc(4));  // [0] bad, [1] K-th position, [2] longer count, [3] max length
+    CEL_TRY(S.rk0.reserve(nnz));
+    CEL_TRY(S.rk1.reserve(nnz));
+    CEL_TRY(S.pay0.reserve(nnz));
+    CEL_TRY(S.pay1.reserve(nnz));
+    CEL_TRY(S.lists.reserve(4));  // [0] bad, [1] K-th position, [2] longer count, [3] max length
     CEL_HIP(hipMemsetAsync(S.lists.p, 0, 4 * sizeof(int), st));
     k_pack<<<g, kThreads, 0, st>>>(nnz, ri, ci, vi, rbase, nrows, cbase, ncols, S.rk0.p, S.pay0.p,
                                     S.lists.p);
@@ -433,9 +461,9 @@ int sort_device(SortBufs& S, int nrows, int ncols, long long nnz, int rbase, int
     };
 
     // groups that are column-sorted
-    CEL_TRY(S.a.alloc(nnz));
-    CEL_TRY(S.b.alloc(nnz));
-    CEL_TRY(S.c.alloc(nnz));
+    CEL_TRY(S.a.reserve(nnz));
+    CEL_TRY(S.b.reserve(nnz));
+    CEL_TRY(S.c.reserve(nnz));
     int* mark = S.c.p;
     if (mode == CEL_SORT_ROW_COL) {
         k_row_ends<<<g, kThreads, 0, st>>>(nnz, S.rk1.p, mark);
@@ -468,7 +496,7 @@ int sort_device(SortBufs& S, int nrows, int ncols, long long nnz, int rbase, int
     CEL_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, head, ih, n, st));
     CEL_TRY(temp_fit(S, bytes));
     CEL_HIP(hipcub::DeviceScan::InclusiveSum(S.temp.p, bytes, head, ih, n, st));
-    CEL_TRY(S.starts.alloc(nnz + 1));
+    CEL_TRY(S.starts.reserve(nnz + 1));
     k_starts<<<g, kThreads, 0, st>>>(nnz, head, ih, S.starts.p);
     CEL_HIP(hipGetLastError());
     int G = 0;
@@ -494,10 +522,10 @@ int sort_device(SortBufs& S, int nrows, int ncols, long long nnz, int rbase, int
     // (group, column) over everything
     const int cb = bits_for(ncols > 0 ? (unsigned long long)(ncols - 1) : 0ULL);
     const int sb = bits_for((unsigned long long)G);
-    CEL_TRY(S.k0.alloc(nnz));
-    CEL_TRY(S.k1.alloc(nnz));
-    CEL_TRY(S.idx.alloc(nnz));
-    CEL_TRY(S.perm.alloc(nnz));
+    CEL_TRY(S.k0.reserve(nnz));
+    CEL_TRY(S.k1.reserve(nnz));
+    CEL_TRY(S.idx.reserve(nnz));
+    CEL_TRY(S.perm.reserve(nnz));
     k_group_keys<<<g, kThreads, 0, st>>>(nnz, ih, S.pay1.p, cb, S.k0.p, S.idx.p);
     CEL_HIP(hipGetLastError());
     CEL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, S.k0.p, S.k1.p, S.idx.p, S.perm.p, n,
@@ -525,24 +553,25 @@ bool bad_common(int nrows, int ncols, int nnz, int index_base, int mode, int mem
 int row_offsets_device(int nrows, long long nnz, const int* rows, int rbase, int out_base, int* xadj)
 {
     hipStream_t st = rt().stream;
-    DevBuf<int> flag, cnt;
-    CEL_TRY(flag.alloc(1));
-    CEL_TRY(cnt.alloc((size_t)nrows + 1));
-    CEL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), st));
-    k_fill_int<<<grid_for(nrows + 1LL), kThreads, 0, st>>>(nrows + 1LL, cnt.p, 0);
-    k_fill_int<<<1, 1, 0, st>>>(1, cnt.p, out_base);  // xadj[0] = out_base carries through the scan
+    SortBufs& S = sort_cache();
+    CEL_TRY(S.flag.reserve(1));
+    CEL_TRY(S.cnt.reserve((size_t)nrows + 1));
+    int* flag = S.flag.p;
+    int* cnt = S.cnt.p;
+    CEL_HIP(hipMemsetAsync(flag, 0, sizeof(int), st));
+    k_fill_int<<<grid_for(nrows + 1LL), kThreads, 0, st>>>(nrows + 1LL, cnt, 0);
+    k_fill_int<<<1, 1, 0, st>>>(1, cnt, out_base);  // xadj[0] = out_base carries through the scan
     if (nnz > 0) {
-        k_rows_in_range<<<grid_for(nnz), kThreads, 0, st>>>(nnz, rows, rbase, nrows, flag.p);
-        k_row_count<<<grid_for(nnz), kThreads, 0, st>>>(nnz, rows, rbase, nrows, cnt.p);
+        k_rows_in_range<<<grid_for(nnz), kThreads, 0, st>>>(nnz, rows, rbase, nrows, flag);
+        k_row_count<<<grid_for(nnz), kThreads, 0, st>>>(nnz, rows, rbase, nrows, cnt);
     }
     CEL_HIP(hipGetLastError());
     int bad = 0;
-    CEL_HIP(hipMemcpyAsync(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    CEL_HIP(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, st));
     size_t bytes = 0;
-    CEL_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, cnt.p, xadj, nrows + 1, st));
-    DevBuf<unsigned char> temp;
-    CEL_TRY(temp.alloc(bytes ? bytes : 1));
-    CEL_HIP(hipcub::DeviceScan::InclusiveSum(temp.p, bytes, cnt.p, xadj, nrows + 1, st));
+    CEL_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, cnt, xadj, nrows + 1, st));
+    CEL_TRY(temp_fit(S, bytes));
+    CEL_HIP(hipcub::DeviceScan::InclusiveSum(S.temp.p, bytes, cnt, xadj, nrows + 1, st));
     CEL_HIP(hipStreamSynchronize(st));
     if (bad) return fail(CEL_ERR_BAD_MATRIX, "row index out of range");
     return CEL_OK;
@@ -552,6 +581,8 @@ int launch_coo_to_csr(int nrows, int nnz, const int* rows0, int* xadj)
 {
     return row_offsets_device(nrows, nnz, rows0, 0, 0, xadj);
 }
+
+void coo_release_workspace() { sort_cache().release(); }
 
 }  // namespace cel
 
@@ -572,26 +603,26 @@ extern "C" int cel_gpu_coo_sort(int nrows, int ncols, int nnz, int* rows, int* c
     }
     const int rbase = index_base + row_offset, cbase = index_base + col_offset;
     hipStream_t st = rt().stream;
-    SortBufs S;
+    SortBufs& S = sort_cache();
     int rc;
     if (memspace == CEL_MEM_DEVICE) {
         rc = sort_device(S, nrows, ncols, nnz, rbase, cbase, row_offset, mode, rows, cols, vals, rows,
                          cols, vals);
     } else {
-        DevBuf<int> dr, dc;
-        DevBuf<double> dv;
-        CEL_TRY(dr.alloc(nnz));
-        CEL_TRY(dc.alloc(nnz));
-        CEL_TRY(dv.alloc(nnz));
-        CEL_HIP(hipMemcpyAsync(dr.p, rows, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice, st));
-        CEL_HIP(hipMemcpyAsync(dc.p, cols, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice, st));
-        CEL_HIP(hipMemcpyAsync(dv.p, vals, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice, st));
-        rc = sort_device(S, nrows, ncols, nnz, rbase, cbase, row_offset, mode, dr.p, dc.p, dv.p, dr.p,
-                         dc.p, dv.p);
+        CEL_TRY(S.rows.reserve(nnz));
+        CEL_TRY(S.cols.reserve(nnz));
+        CEL_TRY(S.vals.reserve(nnz));
+        int* dr = S.rows.p;
+        int* dc = S.cols.p;
+        double* dv = S.vals.p;
+        CEL_HIP(hipMemcpyAsync(dr, rows, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice, st));
+        CEL_HIP(hipMemcpyAsync(dc, cols, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice, st));
+        CEL_HIP(hipMemcpyAsync(dv, vals, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice, st));
+        rc = sort_device(S, nrows, ncols, nnz, rbase, cbase, row_offset, mode, dr, dc, dv, dr, dc, dv);
         if (rc == CEL_OK || rc == CEL_ERR_REF_UNDEFINED) {
-            CEL_HIP(hipMemcpyAsync(rows, dr.p, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost, st));
-            CEL_HIP(hipMemcpyAsync(cols, dc.p, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost, st));
-            CEL_HIP(hipMemcpyAsync(vals, dv.p, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost, st));
+            CEL_HIP(hipMemcpyAsync(rows, dr, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost, st));
+            CEL_HIP(hipMemcpyAsync(cols, dc, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost, st));
+            CEL_HIP(hipMemcpyAsync(vals, dv, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost, st));
         }
         CEL_HIP(hipStreamSynchronize(st));
     }
@@ -607,17 +638,17 @@ extern "C" int cel_gpu_coo_row_offsets(int nrows, int nnz, const int* rows, int 
         (nnz > 0 && !rows) || (memspace != CEL_MEM_HOST && memspace != CEL_MEM_DEVICE))
         return fail(CEL_ERR_ARG, "bad arguments to cel_gpu_coo_row_offsets");
     hipStream_t st = rt().stream;
-    DevBuf<int> rin, dx;
+    SortBufs& S = sort_cache();
     const int* r = rows;
     if (memspace == CEL_MEM_HOST && nnz > 0) {
-        CEL_TRY(rin.alloc(nnz));
-        CEL_HIP(hipMemcpyAsync(rin.p, rows, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice, st));
-        r = rin.p;
+        CEL_TRY(S.rows.reserve(nnz));
+        CEL_HIP(hipMemcpyAsync(S.rows.p, rows, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice, st));
+        r = S.rows.p;
     }
     int* x = xadj;
     if (memspace == CEL_MEM_HOST) {
-        CEL_TRY(dx.alloc((size_t)nrows + 1));
-        x = dx.p;
+        CEL_TRY(S.xadj.reserve((size_t)nrows + 1));
+        x = S.xadj.p;
     }
     CEL_TRY(row_offsets_device(nrows, nnz, r, index_base + row_offset, index_base, x));
     if (memspace == CEL_MEM_HOST)
@@ -639,13 +670,16 @@ extern "C" int cel_gpu_mat_create_coo_sorted(cel_gpu_mat** A, int nrows, int nco
         return fail(CEL_ERR_REF_UNDEFINED, "no entries: the reference's column loop reads rows(1)");
     hipStream_t st = rt().stream;
     const int rbase = index_base + row_offset, cbase = index_base + col_offset;
-    SortBufs S;
-    DevBuf<int> ro, co, xadj, flag;
-    DevBuf<double> vo;
-    CEL_TRY(ro.alloc(nnz > 0 ? nnz : 1));
-    CEL_TRY(co.alloc(nnz > 0 ? nnz : 1));
-    CEL_TRY(vo.alloc(nnz > 0 ? nnz : 1));
-    CEL_TRY(xadj.alloc((size_t)nrows + 1));
+    SortBufs& S = sort_cache();
+    DevBuf<int>& ro = S.rows;
+    DevBuf<int>& co = S.cols;
+    DevBuf<int>& xadj = S.xadj;
+    DevBuf<int>& flag = S.flag;
+    DevBuf<double>& vo = S.vals;
+    CEL_TRY(ro.reserve(nnz > 0 ? nnz : 1));
+    CEL_TRY(co.reserve(nnz > 0 ? nnz : 1));
+    CEL_TRY(vo.reserve(nnz > 0 ? nnz : 1));
+    CEL_TRY(xadj.reserve((size_t)nrows + 1));
     if (nnz > 0) {
         const int* ri = rows;
         const int* ci = cols;
@@ -662,7 +696,7 @@ extern "C" int cel_gpu_mat_create_coo_sorted(cel_gpu_mat** A, int nrows, int nco
                             co.p, vo.p));
         // a CSR build needs rows that never decrease; the reference's sort
         // can leave a last row of one entry inside the row before it
-        CEL_TRY(flag.alloc(1));
+        CEL_TRY(flag.reserve(1));
         CEL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), st));
         k_rows_monotone<<<grid_for(nnz), kThreads, 0, st>>>(nnz, ro.p, flag.p);
         CEL_HIP(hipGetLastError());

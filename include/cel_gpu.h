@@ -210,7 +210,11 @@ int cel_gpu_mm_read(const char* path, int flags, int cap, int* rows, int* cols, 
  * not 1 stays first, and a last row of one entry is column-sorted together
  * with the row before it.  Where its loop reads past the arrays (fewer row
  * changes than nrows-1, e.g. an empty row) the arrays are left sorted by
- * row and CEL_ERR_REF_UNDEFINED is returned. */
+ * row and CEL_ERR_REF_UNDEFINED is returned.  The sort keeps its device
+ * workspace (about 50 bytes per entry; up to 90 with host arrays or column
+ * groups too long for the in-LDS sort; shared with cel_gpu_coo_row_offsets
+ * and cel_gpu_mat_create_coo_sorted) from call to call until
+ * cel_gpu_finalize; calls that overlap from several host threads race. */
 int cel_gpu_coo_sort(int nrows, int ncols, int nnz, int* rows, int* cols, double* vals,
                      int index_base, int row_offset, int col_offset, int mode, int memspace);
 /* cel_sp_mat_to_csr (src/fspmt/cel_sp_mat_module.f90:430-456): xadj from
