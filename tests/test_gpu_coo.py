@@ -136,6 +136,28 @@ def test_shuffled_poisson_full_size(gpu):
     np.testing.assert_array_equal(c[:last], A.cols[:last])
 
 
+def test_workspace_kept_between_sorts_of_changing_size(gpu):
+    """The sort keeps its device workspace between calls: a large sort, a
+    small one (the workspace holds the large one's leftovers), a larger one
+    again (the workspace grows), and the small one repeated; each equals the
+    oracle entry for entry, in both orders."""
+    rng = np.random.default_rng(35)
+    big = O.gen_poisson7(40, 40, 40)
+    bigger = O.gen_poisson7(48, 48, 48)
+    small = O.gen_poisson7(5, 6, 7)
+    for A in (big, small, bigger, small):
+        tag = np.arange(A.nnz, dtype=np.float64)
+        rows, cols, perm = shuffled_coo(O.CSR(A.xadj, A.cols, tag), rng)
+        print(f"n={A.n} nnz={A.nnz}", flush=True)
+        st0, r0, c0, v0 = O.coo_sort_cel(A.n, rows, cols, perm)
+        st, r, c, v = gpu.coo_sort(A.n, rows, cols, perm)
+        assert st == st0 == 0
+        _check_same((r, c, v), (r0, c0, v0))
+        _, r, c, v = gpu.coo_sort(A.n, rows, cols, perm, mode=gpu.SORT_ROW_COL)
+        want_rows = np.repeat(np.arange(1, A.n + 1), np.diff(A.xadj))
+        _check_same((r, c, v), (want_rows, A.cols, tag))
+
+
 def test_create_coo_sorted_matches_oracle_pipeline(gpu):
     """Sort + cel_sp_mat_to_csr + build in HBM, then SpMV, equals the
     oracle's sort + to_csr + CSR SpMV bitwise (both modes)."""
