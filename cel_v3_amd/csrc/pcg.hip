@@ -947,7 +947,7 @@ static int build_hrow_tile(const Mat& A, Workspace& W, int ntiles)
         while (q < A.halo_nrows && hrows[q] < t * kTileRows) ++q;
         ht[t] = q;
     }
-    CEL_TRY(W.hrow_tile.alloc(ntiles + 1));
+    CEL_TRY(W.hrow_tile.reserve(ntiles + 1));
     CEL_HIP(hipMemcpy(W.hrow_tile.p, ht.data(), sizeof(int) * (ntiles + 1), hipMemcpyHostToDevice));
     return CEL_OK;
 }
@@ -1046,10 +1046,10 @@ int pcg_spmv_dot(Mat& A, int fmt, const double* x, const double* ghost, double* 
                                       ntiles),
                           false);
     Workspace& W = ws_of(A);
-    CEL_TRY(W.part1.alloc((size_t)G1));
-    CEL_TRY(W.ctrl.alloc(5));
+    CEL_TRY(W.part1.reserve((size_t)G1));
+    CEL_TRY(W.ctrl.reserve(5));
     if (halo) {
-        CEL_TRY(W.hrec.alloc(2 * (size_t)std::max(A.halo_nrows, 1)));
+        CEL_TRY(W.hrec.reserve(2 * (size_t)std::max(A.halo_nrows, 1)));
         // the CSR K1 finds the entries of rows with more than kHaloRec halo
         // entries through the per-tile compact row ranges
         CEL_TRY(build_hrow_tile(A, W, ntiles));
@@ -1241,10 +1241,10 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         w.ctrl = reinterpret_cast<Ctrl*>(base + kNoAllocDoubles);
     } else {
         Workspace& W = *Wp;
-        CEL_TRY(W.part1.alloc((size_t)G1));
-        CEL_TRY(W.part2.alloc(2 * (size_t)Gv));
-        CEL_TRY(W.rpart.alloc(3 * (size_t)Gr));
-        CEL_TRY(W.ctrl.alloc(5));
+        CEL_TRY(W.part1.reserve((size_t)G1));
+        CEL_TRY(W.part2.reserve(2 * (size_t)Gv));
+        CEL_TRY(W.rpart.reserve(3 * (size_t)Gr));
+        CEL_TRY(W.ctrl.reserve(5));
         if (mixed) {
             CEL_TRY(W.r32.alloc(nn));
             CEL_TRY(W.p32.alloc(nn));
@@ -1298,7 +1298,7 @@ int pcg_solve(Mat& A, const double* b, double* x, const cel_pcg_params& prm, cel
         CEL_TRY(W.ghost.alloc(A.ghost_len > 0 ? A.ghost_len : 1));
         CEL_TRY(W.sendbuf.alloc(A.send_total > 0 ? A.send_total : 1));
         hipLaunchKernelGGL(k_fill, dim3(64), dim3(kBlock), 0, s, (long long)W.ghost.n, W.ghost.p, 0.0);
-        CEL_TRY(W.hrec.alloc(2 * (size_t)std::max(A.halo_nrows, 1)));
+        CEL_TRY(W.hrec.reserve(2 * (size_t)std::max(A.halo_nrows, 1)));
         CEL_TRY(build_hrow_tile(A, W, ntiles));
     }
 
