@@ -67,7 +67,8 @@ struct BSys {
     long long voff;  // first element of the system's vectors
     int n, kp, ksh;  // rows, padded lanes (power of two), log2(kp)
     int lane0;       // first lane of the system
-    int b0, b1;      // current block range
+    int b0, b1;      // current block range of K1 and the set-up (their table)
+    int c0, c1;      // current block range of K2 and K3 (their own table and grid)
     int ntiles;      // tiles of 256/kp rows
     int period;      // multi-rhs: band period in 256-row chunks (0: none)
 };
@@ -216,10 +217,12 @@ __device__ __forceinline__ void read_lanes(const BCtrl* __restrict__ cin, const 
     __syncthreads();
 }
 
-__device__ __forceinline__ void write_lanes(BCtrl* __restrict__ cout, const BSys& S, int b,
+// The first block of the system's range (b0 for K1, c0 for K2/K3) writes
+// its lanes' tokens.
+__device__ __forceinline__ void write_lanes(BCtrl* __restrict__ cout, const BSys& S, bool first,
                                             const BCtrl* lc)
 {
-    if (b == S.b0 && threadIdx.x < S.kp) cout[S.lane0 + threadIdx.x] = lc[threadIdx.x];
+    if (first && threadIdx.x < S.kp) cout[S.lane0 + threadIdx.x] = lc[threadIdx.x];
 }
 
 // ---------------------------------------------------------------- set-up
@@ -324,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
     }
     __syncthreads();
     if (!any) {
-        write_lanes(cout, S, b, lc);
+        write_lanes(cout, S, b == S.b0, lc);
         return;
     }
     const double* ps = p + S.voff;
@@ -545,12 +548,12 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BSys* __restrict__ sys,
         }
         const double bs = lane_pair_block_sum(acc, acc1, S.kp, red);
         if (threadIdx.x < S.kp) part[(size_t)b * kpmax + threadIdx.x] = bs;
-        write_lanes(cout, S, b, lc);
+        write_lanes(cout, S, b == S.b0, lc);
         return;
     }
     const double bs = lane_block_sum(acc, S.kp, red);
     if (threadIdx.x < S.kp) part[(size_t)b * kpmax + threadIdx.x] = bs;
-    write_lanes(cout, S, b, lc);
+    write_lanes(cout, S, b == S.b0, lc);
 }
 
 // ---------------------------------------------------------------- K2
@@ -580,9 +583,10 @@ __global__ __launch_bounds__(kBlock) void k_bupdate_r(const BSys* __restrict__ s
     }
     __syncthreads();
     if (!any) {
-        write_lanes(cout, S, b, lc);
+        write_lanes(cout, S, b == S.c0, lc);
         return;
     }
+    // K1's partials: its blocks [b0, b1) of the K1 table
     lane_partials(vpart, kpmax, S.b0, S.b1, S.kp, S.ksh, s1, red);
     if (threadIdx.x < S.kp) {
         BCtrl& c = lc[threadIdx.x];
@@ -646,7 +650,7 @@ __global__ __launch_bounds__(kBlock) void k_bupdate_r(const BSys* __restrict__ s
         part[(size_t)b * kpmax + threadIdx.x] = a1;
         part[((size_t)G + b) * kpmax + threadIdx.x] = a2;
     }
-    write_lanes(cout, S, b, lc);
+    write_lanes(cout, S, b == S.c0, lc);
 }
 
 // ---------------------------------------------------------------- K3
@@ -677,11 +681,12 @@ __global__ __launch_bounds__(kBlock) void k_bupdate_xp(const BSys* __restrict__ 
     }
     __syncthreads();
     if (!any) {
-        write_lanes(cout, S, b, lc);
+        write_lanes(cout, S, b == S.c0, lc);
         return;
     }
-    lane_partials(rpart, kpmax, S.b0, S.b1, S.kp, S.ksh, s1, red);
-    lane_partials(rpart + (size_t)G * kpmax, kpmax, S.b0, S.b1, S.kp, S.ksh, s2, red);
+    // K2's partials: its blocks [c0, c1) of its table (G of them)
+    lane_partials(rpart, kpmax, S.c0, S.c1, S.kp, S.ksh, s1, red);
+    lane_partials(rpart + (size_t)G * kpmax, kpmax, S.c0, S.c1, S.kp, S.ksh, s2, red);
     if (threadIdx.x < S.kp) sb[threadIdx.x] = s1[threadIdx.x] / lc[threadIdx.x].alpha;
     __syncthreads();
     const int j = threadIdx.x & (S.kp - 1);
@@ -731,7 +736,7 @@ __global__ __launch_bounds__(kBlock) void k_bupdate_xp(const BSys* __restrict__ 
         }
     }
     __syncthreads();
-    write_lanes(cout, S, b, lc);
+    write_lanes(cout, S, b == S.c0, lc);
 }
 
 // ---------------------------------------------------------------- layout kernels
@@ -770,10 +775,12 @@ __global__ void k_bfill(double* __restrict__ dst, int n, int k, int kp, int ksh,
 
 // ---------------------------------------------------------------- host side
 struct Tuning {
-    int grid;     // blocks of the persistent grid
+    int grid;     // blocks of K1 and the set-up
     int chunk;    // iterations per graph replay
     int weight;   // 0: blocks by entries x lanes, 1: by rows x lanes
     int rebalance;
+    int grid23 = 0;  // blocks of K2 and K3 (0: grid)
+    int g23() const { return grid23 > 0 ? grid23 : grid; }
 };
 
 struct Batch {
@@ -785,7 +792,7 @@ struct Batch {
     int lanes = 0;        // padded lanes
     int kpmax = 1;
     DevBuf<BSys> d_sys;
-    DevBuf<BBlk> d_tbl;
+    DevBuf<BBlk> d_tbl, d_tbl2;  // K1 and set-up; K2 and K3
     DevBuf<double> r, p, v, x, b, part1, part2, rpart;
     DevBuf<BCtrl> ctrl;  // 3 slots of `lanes` tokens
     std::vector<BSys> h_sys;
@@ -806,8 +813,10 @@ static int grid_max()
 // Deals G blocks to the systems with active[s] (at least one each, when
 // there are enough blocks), proportional to their work, and fills the work
 // table.  Blocks of the table are indexed by logical block.
+// Deals G blocks to the active systems (ranges b0/b1 of the K1 table, or
+// c0/c1 of the K2/K3 table when k23).
 static int build_table(Batch& B, int G, int weight, const std::vector<char>& active,
-                       std::vector<BBlk>& tbl)
+                       std::vector<BBlk>& tbl, bool k23)
 {
     const int S = (int)B.mats.size();
     std::vector<double> w(S, 0.0);
@@ -822,7 +831,10 @@ static int build_table(Batch& B, int G, int weight, const std::vector<char>& act
     }
     tbl.assign(G, BBlk{-1, 0, 0, 0, 0});
     for (int s = 0; s < S; ++s) {
-        B.h_sys[s].b0 = B.h_sys[s].b1 = 0;
+        if (k23)
+            B.h_sys[s].c0 = B.h_sys[s].c1 = 0;
+        else
+            B.h_sys[s].b0 = B.h_sys[s].b1 = 0;
     }
     if (nact == 0) return CEL_OK;
     if (nact > G) return fail(CEL_ERR_ARG, "more systems than resident blocks in one batch");
@@ -855,8 +867,8 @@ static int build_table(Batch& B, int G, int weight, const std::vector<char>& act
     for (int s = 0; s < S; ++s) {
         if (!active[s]) continue;
         BSys& H = B.h_sys[s];
-        H.b0 = cur;
-        H.b1 = cur + nb[s];
+        (k23 ? H.c0 : H.b0) = cur;
+        (k23 ? H.c1 : H.b1) = cur + nb[s];
         const long long units = (((long long)H.n << H.ksh) + kBlock - 1) / kBlock;
         for (int i = 0; i < nb[s]; ++i) {
             BBlk& e = tbl[cur + i];
@@ -871,10 +883,14 @@ static int build_table(Batch& B, int G, int weight, const std::vector<char>& act
     return CEL_OK;
 }
 
-static int upload_table(Batch& B, const std::vector<BBlk>& tbl, hipStream_t s)
+static int upload_table(Batch& B, const std::vector<BBlk>& tbl, const std::vector<BBlk>& tbl2,
+                        hipStream_t s)
 {
     CEL_TRY(B.d_tbl.alloc(tbl.size()));
+    CEL_TRY(B.d_tbl2.alloc(tbl2.size()));
     CEL_HIP(hipMemcpyAsync(B.d_tbl.p, tbl.data(), sizeof(BBlk) * tbl.size(), hipMemcpyHostToDevice, s));
+    CEL_HIP(hipMemcpyAsync(B.d_tbl2.p, tbl2.data(), sizeof(BBlk) * tbl2.size(), hipMemcpyHostToDevice,
+                           s));
     CEL_HIP(hipMemcpyAsync(B.d_sys.p, B.h_sys.data(), sizeof(BSys) * B.h_sys.size(),
                            hipMemcpyHostToDevice, s));
     // the copies read pageable host memory: finish them before the host
@@ -895,15 +911,17 @@ static int run_batch(Batch& B, const Tuning& T, const cel_pcg_params& prm, int i
 {
     hipStream_t s = rt().stream;
     const int G = T.grid;
+    const int G2 = T.g23();
     const int S = (int)B.mats.size();
     CEL_TRY(B.part1.alloc((size_t)G * B.kpmax));
-    CEL_TRY(B.part2.alloc((size_t)2 * G * B.kpmax));
+    CEL_TRY(B.part2.alloc((size_t)2 * G2 * B.kpmax));
     CEL_TRY(B.rpart.alloc((size_t)3 * G * B.kpmax));
     CEL_TRY(B.ctrl.alloc((size_t)3 * B.lanes));
     std::vector<char> active(S, 1);
-    std::vector<BBlk> tbl;
-    CEL_TRY(build_table(B, G, T.weight, active, tbl));
-    CEL_TRY(upload_table(B, tbl, s));
+    std::vector<BBlk> tbl, tbl2;
+    CEL_TRY(build_table(B, G, T.weight, active, tbl, false));
+    CEL_TRY(build_table(B, G2, T.weight, active, tbl2, true));
+    CEL_TRY(upload_table(B, tbl, tbl2, s));
 
     BPrm P{};
     P.tol = prm.tol;
@@ -928,10 +946,10 @@ static int run_batch(Batch& B, const Tuning& T, const cel_pcg_params& prm, int i
     auto enqueue_iter = [&]() -> int {
         hipLaunchKernelGGL(k_bspmv, dim3(G), dim3(kBlock), 0, s, B.d_sys.p, B.d_tbl.p, B.p.p, B.v.p,
                            B.part1.p, B.rpart.p, B.kpmax, G, C0, C1, P);
-        hipLaunchKernelGGL(k_bupdate_r, dim3(G), dim3(kBlock), 0, s, B.d_sys.p, B.d_tbl.p, B.r.p,
-                           B.v.p, B.part2.p, B.part1.p, B.kpmax, G, C1, C2);
-        hipLaunchKernelGGL(k_bupdate_xp, dim3(G), dim3(kBlock), 0, s, B.d_sys.p, B.d_tbl.p, B.x.p,
-                           B.p.p, B.r.p, B.part2.p, B.kpmax, G, C2, C0, P);
+        hipLaunchKernelGGL(k_bupdate_r, dim3(G2), dim3(kBlock), 0, s, B.d_sys.p, B.d_tbl2.p, B.r.p,
+                           B.v.p, B.part2.p, B.part1.p, B.kpmax, G2, C1, C2);
+        hipLaunchKernelGGL(k_bupdate_xp, dim3(G2), dim3(kBlock), 0, s, B.d_sys.p, B.d_tbl2.p, B.x.p,
+                           B.p.p, B.r.p, B.part2.p, B.kpmax, G2, C2, C0, P);
         CEL_HIP(hipGetLastError());
         return CEL_OK;
     };
@@ -981,9 +999,10 @@ static int run_batch(Batch& B, const Tuning& T, const cel_pcg_params& prm, int i
                 // in slot 0; the chunk in flight does not touch it again
                 CEL_HIP(hipEventSynchronize(cev[k & 1]));
                 active = act;
-                rc = build_table(B, G, T.weight, active, tbl);
+                rc = build_table(B, G, T.weight, active, tbl, false);
+                if (rc >= 0) rc = build_table(B, G2, T.weight, active, tbl2, true);
                 if (rc < 0) break;
-                rc = upload_table(B, tbl, s);
+                rc = upload_table(B, tbl, tbl2, s);
                 if (rc < 0) break;
             }
         }
@@ -1195,14 +1214,21 @@ extern "C" int cel_gpu_batch_tune(cel_gpu_batch* h, const double* const* b, int 
     prm.x0_mode = CEL_X0_ZERO;
     prm.alpha_stop = 0;
     const int gmax = std::max(kXcds, grid_max() / kXcds * kXcds);
+    // K1 on all, three quarters or half of the resident grid; K2 and K3
+    // on the same grid or on half of the resident grid
     std::vector<Tuning> cand;
-    for (int gdiv : {1, 2})
-        for (int weight : {0, 1})
-            for (int chunk : {8, 32}) {
-                int g = std::max(kXcds, gmax / gdiv / kXcds * kXcds);
-                if (g < (int)B.mats.size()) continue;
-                cand.push_back(Tuning{g, chunk, weight, 1});
-            }
+    for (int gq : {4, 3, 2})
+        for (int g2q : {0, 2})
+            for (int weight : {0, 1})
+                for (int chunk : {8, 32}) {
+                    const int g = std::max(kXcds, gmax * gq / 4 / kXcds * kXcds);
+                    const int g2 = g2q ? std::max(kXcds, gmax * g2q / 4 / kXcds * kXcds) : g;
+                    if (g < (int)B.mats.size() || g2 < (int)B.mats.size()) continue;
+                    if (g2q && g2 == g) continue;  // the same as g2q = 0
+                    Tuning T{g, chunk, weight, 1};
+                    T.grid23 = g2;
+                    cand.push_back(T);
+                }
     double best = 1e300;
     Tuning bt = B.tune;
     for (const Tuning& T : cand) {
