@@ -404,11 +404,12 @@ int cel_gpu_diffusion7_kappa_grad(int nx, int ny, int nz, const double* kappa, c
 typedef struct cel_gpu_batch cel_gpu_batch;
 int cel_gpu_batch_create(cel_gpu_batch** B, int nsys, cel_gpu_mat* const* mats, const int* nrhs);
 int cel_gpu_batch_destroy(cel_gpu_batch* B);
-/* Times candidate launch configurations (grid size, iterations per graph
- * replay, block apportionment) on `iters` iterations of the batch with the
- * right-hand sides b and keeps the fastest for later solves.  The choice
- * changes how dot products are grouped, so results agree with an untuned
- * solve to rounding, not bit for bit. */
+/* Times candidate launch configurations (the grid of the SpMV, the grid of
+ * the two vector kernels, iterations per graph replay, block apportionment)
+ * on `iters` iterations of the batch with the right-hand sides b and keeps
+ * the fastest for later solves.  The choice changes how dot products are
+ * grouped, so results agree with an untuned solve to rounding, not bit for
+ * bit. */
 int cel_gpu_batch_tune(cel_gpu_batch* B, const double* const* b, int memspace, int iters);
 /* The configuration in use and the time per iteration the tuner measured
  * (0 before tuning).  weight 0: blocks dealt by entries x lanes, 1: by rows x lanes. */
