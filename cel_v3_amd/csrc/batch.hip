@@ -15,12 +15,14 @@
 // two lanes each) and the matrix entries of a row are read once for all kp
 // lanes; the vector kernels cover 256 elements per block and step.
 //
-// Work table.  Every block of the persistent grid is assigned to one
-// system (a contiguous block range per system, sized by the system's work)
-// and owns a contiguous range of that system's tiles/elements.  Partial
-// dot products are per (block, lane); the consumer kernel sums the
-// partials of the system's block range in block order, so all lane scalars
-// are deterministic for a given table.  The table lives in device memory:
+// Work tables.  Every block of a kernel's grid is assigned to one system
+// (a contiguous block range per system, sized by the system's work) and
+// owns a contiguous range of that system's tiles/elements.  K1 and the
+// set-up share one table (ranges b0/b1); K2 and K3 share one of their own,
+// over a grid of their own (ranges c0/c1).  Partial dot products are per
+// (block, lane); the consumer kernel sums the producer's partials of the
+// system's block range in block order, so all lane scalars are
+// deterministic for given tables.  The tables live in device memory:
 // between chunks of iterations the host re-deals the blocks of systems that
 // have stopped to the systems still running, without re-capturing the
 // hipGraph that replays the chunk.
