@@ -49,6 +49,7 @@ EXPORTS = [
     "cel_gpu_spmv_dot",
     "cel_gpu_batch_create",
     "cel_gpu_batch_destroy", "cel_gpu_batch_tune", "cel_gpu_batch_info", "cel_gpu_batch_solve",
+    "cel_gpu_batch_get_config", "cel_gpu_batch_set_config",
     "cel_gpu_mm_info", "cel_gpu_mm_read", "cel_gpu_coo_sort", "cel_gpu_coo_row_offsets",
     "cel_gpu_mat_create_coo_sorted", "cel_gpu_gmres_params_default", "cel_gpu_gmres_solve",
     "cel_gpu_elm_create", "cel_gpu_elm_row_start", "cel_gpu_elm_insert", "cel_gpu_elm_reinsert",
@@ -542,9 +543,16 @@ class Batch:
         _check(lib().cel_gpu_batch_info(self.h, C.byref(lanes), C.byref(el), C.byref(grid),
                                         C.byref(chunk), C.byref(weight), C.byref(reb),
                                         C.byref(ms)))
+        g23 = C.c_int()
+        _check(lib().cel_gpu_batch_get_config(self.h, None, C.byref(g23), None, None))
         return {"lanes": lanes.value, "elements": el.value, "grid": grid.value,
-                "chunk": chunk.value, "weight": weight.value, "rebalance": reb.value,
-                "ms_per_iter": ms.value}
+                "grid23": g23.value, "chunk": chunk.value, "weight": weight.value,
+                "rebalance": reb.value, "ms_per_iter": ms.value}
+
+    def set_config(self, grid, grid23=0, chunk=32, weight=0):
+        """Fixes the launch configuration of later solves (cel_gpu_batch_set_config)."""
+        _check(lib().cel_gpu_batch_set_config(self.h, int(grid), int(grid23), int(chunk),
+                                              int(weight)))
 
     def solve(self, bs, x0s=None, tol=1e-9, tol_mode=TOL_REL_B, iter_max=10000, x0_mode=X0_ZERO,
               allow_not_converged=True, **kw):

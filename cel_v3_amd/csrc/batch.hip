@@ -1203,6 +1203,40 @@ extern "C" int cel_gpu_batch_info(const cel_gpu_batch* h, int* lanes, long long*
     return CEL_OK;
 }
 
+extern "C" int cel_gpu_batch_get_config(const cel_gpu_batch* h, int* grid, int* grid23, int* chunk,
+                                        int* weight)
+{
+    if (!h) return fail(CEL_ERR_ARG, "null batch");
+    const Tuning& T = h->b.tune;
+    if (grid) *grid = T.grid;
+    if (grid23) *grid23 = T.g23();
+    if (chunk) *chunk = T.chunk;
+    if (weight) *weight = T.weight;
+    return CEL_OK;
+}
+
+extern "C" int cel_gpu_batch_set_config(cel_gpu_batch* h, int grid, int grid23, int chunk, int weight)
+{
+    if (!h) return fail(CEL_ERR_ARG, "null batch");
+    CEL_TRY(ensure_init());
+    Batch& B = h->b;
+    const int nsys = (int)B.mats.size();
+    const int gmax = grid_max();
+    if (grid23 == 0) grid23 = grid;
+    for (int g : {grid, grid23})
+        if (g < nsys || g < kXcds || g % kXcds || g > gmax)
+            return fail(CEL_ERR_ARG, "batch grids must be multiples of 8 between the number of "
+                                     "systems and " + std::to_string(gmax));
+    if (chunk < 1 || chunk > 1024 || (weight != 0 && weight != 1))
+        return fail(CEL_ERR_ARG, "batch chunk must be 1..1024 and weight 0 or 1");
+    Tuning T{grid, chunk, weight, 1};
+    T.grid23 = grid23;
+    B.tune = T;
+    B.tuned = false;
+    B.tuned_ms_per_iter = 0.0;
+    return CEL_OK;
+}
+
 extern "C" int cel_gpu_batch_tune(cel_gpu_batch* h, const double* const* b, int memspace, int iters)
 {
     if (!h || !b || iters < 1) return fail(CEL_ERR_ARG, "bad arguments to cel_gpu_batch_tune");

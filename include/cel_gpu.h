@@ -415,6 +415,19 @@ int cel_gpu_batch_tune(cel_gpu_batch* B, const double* const* b, int memspace, i
  * (0 before tuning).  weight 0: blocks dealt by entries x lanes, 1: by rows x lanes. */
 int cel_gpu_batch_info(const cel_gpu_batch* B, int* lanes, long long* elements, int* grid,
                        int* chunk, int* weight, int* rebalance, double* ms_per_iter);
+/* No reference counterpart.  The launch configuration of later solves:
+ * grid = blocks of the SpMV and set-up kernels, grid23 = blocks of the two
+ * vector kernels (their own block table), chunk = iterations per graph
+ * replay, weight as above.  get returns the one in use (after
+ * cel_gpu_batch_tune, the tuner's); set fixes one by hand, so that solves
+ * on different runs or GPUs group their dot products alike (the
+ * configuration changes the grouping, and so the rounding).  set accepts
+ * grids that are multiples of 8, at least the number of systems and at
+ * most the resident grid (grid23 = 0: grid), chunk 1..1024, weight 0 or 1;
+ * anything else is CEL_ERR_ARG. */
+int cel_gpu_batch_get_config(const cel_gpu_batch* B, int* grid, int* grid23, int* chunk,
+                             int* weight);
+int cel_gpu_batch_set_config(cel_gpu_batch* B, int grid, int grid23, int chunk, int weight);
 /* b[s], x[s]: n_s x nrhs[s] column-major (right-hand side j at offset j*n_s),
  * host or device per params->memspace.  lane_stats (may be NULL) receives one
  * entry per right-hand side in system order; total: status (most severe),

@@ -112,6 +112,44 @@ def test_batch_repeat_is_bitwise_and_tuning_keeps_results(gpu):
         np.testing.assert_allclose(xa, xc, rtol=0, atol=1e-8 * np.abs(xa).max())
 
 
+def test_batch_fixed_configurations_match_the_oracle(gpu):
+    """Launch configurations fixed by hand (cel_gpu_batch_set_config): the
+    vector kernels on a grid of their own smaller and larger than the
+    SpMV's, chunks of 1 and 8 iterations, both weightings; every lane equals
+    the oracle to the solver's tolerance, the configuration reads back, and
+    a configuration outside the limits is refused."""
+    mats, bs, nrhs, hosts = small_batch(gpu, seed=16)
+    batch = gpu.Batch(mats, nrhs)
+    gmax = batch.info()["grid"]  # the untuned grid: every resident block
+    assert batch.info()["grid23"] == gmax
+    half = max(8, gmax // 2 // 8 * 8)
+    refs = []
+    for A, b, k in zip(hosts, bs, nrhs):
+        jac = O.set_jacobi(O.csr_diagonal(A))
+        for j in range(k):
+            bj = b if k == 1 else b[j]
+            refs.append(O.pcg_jacobi(A, jac, bj, x0_mode=O.X0_ZERO, eps=1e-9 * np.linalg.norm(bj),
+                                     iter_max=10000, workers=1))
+    for grid, grid23, chunk, weight in ((gmax, half, 8, 0), (half, gmax, 1, 1), (half, 0, 32, 1)):
+        batch.set_config(grid, grid23, chunk, weight)
+        info = batch.info()
+        assert (info["grid"], info["grid23"], info["chunk"], info["weight"]) == \
+            (grid, grid23 or grid, chunk, weight)
+        out = batch.solve(bs, tol=1e-9, tol_mode=gpu.TOL_REL_B, x0_mode=gpu.X0_ZERO, alpha_stop=1)
+        assert out["rc"] == 0, out["total"]
+        xs = lanes_of(out["x"], nrhs)
+        for lane, (ref, st) in enumerate(zip(refs, out["lanes"])):
+            assert st["status"] == 0
+            assert abs(st["iters"] - ref["iters"]) <= 1, (grid, grid23, lane, st["iters"], ref["iters"])
+            if st["iters"] == ref["iters"]:
+                np.testing.assert_allclose(xs[lane], ref["x"], rtol=0,
+                                           atol=1e-10 * np.abs(ref["x"]).max())
+    for bad in ((gmax + 8, 0, 32, 0), (half + 4, 0, 32, 0), (0, 0, 32, 0), (half, 0, 0, 0),
+                (half, 0, 32, 2)):
+        with pytest.raises(gpu.CelError):
+            batch.set_config(*bad)
+
+
 def test_batch_iteration_cap_and_nonfinite_lane(gpu):
     mats, bs, nrhs, _ = small_batch(gpu, seed=14)
     batch = gpu.Batch(mats, nrhs)
