@@ -261,6 +261,20 @@ module cel_gpu
       type(cel_pcg_stats), intent(out) :: lane_stats(*)
       type(cel_pcg_stats), intent(out) :: total
     end function
+    ! the launch configuration of later batch solves (grids of the SpMV and
+    ! of the vector kernels, iterations per graph replay, weighting)
+    integer(c_int) function cel_gpu_batch_get_config(B, grid, grid23, chunk, weight) &
+        bind(c, name="cel_gpu_batch_get_config")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: B
+      integer(c_int), intent(out) :: grid, grid23, chunk, weight
+    end function
+    integer(c_int) function cel_gpu_batch_set_config(B, grid, grid23, chunk, weight) &
+        bind(c, name="cel_gpu_batch_set_config")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: B
+      integer(c_int), value :: grid, grid23, chunk, weight
+    end function
   end interface
 
 contains

@@ -1,13 +1,16 @@
 ! batch_poisson.f90 -- batched Jacobi-PCG through the Fortran bindings:
 ! a 10^3 and a 6x7x8 Poisson system, the second with 3 right-hand sides,
-! solved together; prints per-lane status, iterations and the true
-! relative residual computed with the library's SpMV.
+! solved together on a launch configuration fixed by hand (the vector
+! kernels on a grid of their own); prints the configuration read back,
+! per-lane status, iterations and the true relative residual computed with
+! the library's SpMV.
 program batch_poisson
   use, intrinsic :: iso_c_binding
   use cel_gpu
   implicit none
   type(c_ptr) :: A(2), B
   integer(c_int) :: nrhs(2), ierr, i, j, n(2), lane
+  integer(c_int) :: cg, cg23, cch, cw
   type(c_ptr) :: bp(2), xp(2)
   real(c_double), allocatable, target :: b1(:), x1(:), b2(:), x2(:)
   real(c_double), allocatable :: ax(:)
@@ -35,6 +38,11 @@ program batch_poisson
   end do
   ierr = cel_gpu_batch_create(B, 2_c_int, A, nrhs)
   if (ierr /= CEL_OK) stop 3
+  ierr = cel_gpu_batch_set_config(B, 16_c_int, 8_c_int, 4_c_int, 1_c_int)
+  if (ierr /= CEL_OK) stop 5
+  ierr = cel_gpu_batch_get_config(B, cg, cg23, cch, cw)
+  if (ierr /= CEL_OK) stop 6
+  write (*, '(a,4i6)') 'config ', cg, cg23, cch, cw
   call cel_gpu_pcg_params_default(p)
   p%tol = 1.0d-10
   p%tol_mode = CEL_TOL_REL_B

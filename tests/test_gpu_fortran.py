@@ -59,7 +59,9 @@ def test_fortran_elm_driver(gpu):
 
 def test_fortran_batch_driver(gpu):
     """A 10^3 system and a 6x7x8 system with 3 right-hand sides solved as one
-    batch from Fortran; every lane converges to 1e-10 (true residual from
+    batch from Fortran, on a configuration fixed through the Fortran binding
+    (16 blocks for the SpMV, 8 for the vector kernels, 4 iterations per
+    replay) and read back; every lane converges to 1e-10 (true residual from
     the library's FP64 SpMV) in the iteration count of the single-system
     solver (+-1)."""
     if not os.path.exists(BATCH_EXE):
@@ -69,6 +71,8 @@ def test_fortran_batch_driver(gpu):
     assert out.returncode == 0
     lanes = [ln.split() for ln in out.stdout.splitlines() if ln.startswith("lane")]
     assert len(lanes) == 4
+    config = [ln.split() for ln in out.stdout.splitlines() if ln.startswith("config")]
+    assert config == [["config", "16", "8", "4", "1"]]
     n = 1000
     b1 = 1.0 + (np.arange(1, n + 1) % 7) * 0.25
     M1 = gpu.Matrix.poisson7(10, 10, 10)
