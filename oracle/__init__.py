@@ -1,8 +1,8 @@
 """ctypes front end of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker.  The product library never
-imports this package.
+bench.py's cpu_baseline leg, to check results against.  The product library
+never imports this package.
 
 Every index array that crosses this module is 1-based, exactly as in the
 reference Fortran (xadj[0] == 1).  The C sources (cel_oracle.c) cite the

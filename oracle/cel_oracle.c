@@ -1,7 +1,7 @@
 /*
  * cel_oracle.c -- CPU restatement of the CEL Jacobi-PCG hot path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline); the product
+ * TEST INFRASTRUCTURE ONLY (results to check against, and the CPU baseline); the product
  * library never calls into this file.  See cel_oracle.h for conventions.
  * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
  */

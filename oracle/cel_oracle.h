@@ -1,8 +1,8 @@
 /*
  * cel_oracle.h -- CPU restatement of the CEL Jacobi-PCG hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the GPU
- * library is compared against, and the CPU baseline timed beside it.  Only
+ * TEST INFRASTRUCTURE ONLY.  This library is what the GPU library's results
+ * are compared against, and the CPU baseline timed beside it.  Only
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
  * it.  The product (cel_v3_amd/libcel_gpu.so) never links or calls it.
  *

@@ -1,7 +1,7 @@
 ! ref_mm_bind.f90 -- C entry points into the reference's own Matrix Market
 ! reader and COO sort, compiled from the reference sources into oracle/_ref.
 !
-! TEST INFRASTRUCTURE ONLY: the checker that tests/ compare the oracle's
+! TEST INFRASTRUCTURE ONLY: the results that tests/ compare the oracle's
 ! restatement and the GPU library against.  Nothing here is the product.
 !
 ! ref_mm_info / ref_mm_read make the calls that

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/) before the GPU results are checked against it.
 
 Each oracle function is checked against something that does not restate its
 own formula: the reference's own generator output (tests/golden, from
