@@ -447,7 +447,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              f"(the path's multi-GPU sharding is a design, not part of this build)",
+              f"(the library runs one subdomain on one GPU; sharding across GPUs is not implemented)",
               file=sys.stderr)
         sys.exit(2)
 
