@@ -215,3 +215,44 @@ def test_jad_256_conversion_and_spmv(gpu):
         np.testing.assert_array_equal(j["vals"][begin0[d]:begin0[d + 1]], vals[src])
     x = np.random.default_rng(5).normal(size=M.n)
     np.testing.assert_array_equal(M.spmv(x, fmt=gpu.FMT_JAD), M.spmv(x))
+
+
+@pytest.mark.parametrize("name,A", CASES, ids=[c[0] for c in CASES])
+def test_fused_product_equals_the_oracle_and_refuses_bad_forms(gpu, name, A):
+    """cel_gpu_spmv_dot (the PCG's fused K1 on caller vectors): y bitwise the
+    oracle's row sums, the dot product x.y to rounding (its grouping is the
+    kernel's block tree); a JAD form, ghost values without halo blocks and a
+    DIA form that was never built are refused."""
+    M = dev_matrix(gpu, A)
+    x = np.random.default_rng(5).normal(size=A.n)
+    y, dot = M.spmv_dot(x)
+    want = O.spmv_csr(A, x)
+    np.testing.assert_array_equal(y, want)
+    np.testing.assert_allclose(dot, float(np.dot(x, want)), rtol=1e-12,
+                               atol=1e-12 * float(np.abs(x * want).sum()))
+    with pytest.raises(gpu.CelError):
+        M.spmv_dot(x, fmt=gpu.FMT_JAD)
+    with pytest.raises(gpu.CelError):
+        M.spmv_dot(x, ghost=np.zeros(4))
+    with pytest.raises(gpu.CelError):
+        M.spmv_dot(x, fmt=gpu.FMT_DIA)
+
+
+def test_stream_read_time_is_a_plausible_rate_and_checks_its_arguments(gpu):
+    """cel_gpu_stream_read_time: a 256 MiB read in 1 and 4 arrays runs at
+    between 1 and 9 TB/s (the part's peak is 8 TB/s); out-of-range arguments
+    are refused."""
+    nbytes = 256 << 20
+    for streams in (1, 4):
+        ms = gpu.stream_read_time(nbytes, streams=streams, blocks_per_cu=4, reps=3)
+        rate = nbytes / (ms * 1e-3)
+        print(f"{streams} arrays: {rate / 1e12:.2f} TB/s", flush=True)
+        assert 1e12 < rate < 9e12
+    for bad in (dict(streams=0), dict(streams=9), dict(blocks_per_cu=0), dict(blocks_per_cu=33),
+                dict(reps=0)):
+        kw = dict(streams=1, blocks_per_cu=4, reps=1)
+        kw.update(bad)
+        with pytest.raises(gpu.CelError):
+            gpu.stream_read_time(nbytes, **kw)
+    with pytest.raises(gpu.CelError):
+        gpu.stream_read_time(8, streams=1)
